@@ -1,0 +1,146 @@
+/*
+ * spect_hip.h -- C ABI of libspect_hip.so, the MI355X (gfx950) kernels behind
+ * the SPECT rotate+sum system matrix.
+ *
+ * The reference (PyTomography 3.3.2) is pure Python with no FFI; its
+ * boundary is the duck-typed SystemMatrix / Transform interface.  Each entry
+ * point below replaces one reference call chain, cited as
+ * src/pytomography/<file>:<line>.  The Python drop-in (pytomography_amd)
+ * binds them with ctypes.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative SPECT_E* code; the
+ *    message of the last failure on the calling thread is spect_last_error().
+ *  - All array arguments are device pointers to C-contiguous buffers owned by
+ *    the caller.  Object arrays are [nx][ny][nz] (z fastest), projections are
+ *    [n][ny][nz] (the reference's [Ltheta, Lr, Lz]).
+ *  - dtype selects float32 (SPECT_F32) or float64 (SPECT_F64) for every
+ *    real-valued array of the call.
+ *  - Work is enqueued on `stream` (a hipStream_t, NULL = default stream);
+ *    nothing synchronises the device, so calls can be captured in a hipGraph.
+ *  - A plan is immutable after creation except for the attenuation cache
+ *    (spect_plan_set_mu), so calls with the same plan may run on different
+ *    streams as long as they do not share a workspace.
+ */
+#ifndef SPECT_HIP_H
+#define SPECT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SPECT_OK = 0,
+  SPECT_E_ARG = -1,      /* invalid argument / shape / unsupported combination */
+  SPECT_E_HIP = -2,      /* HIP runtime error */
+  SPECT_E_WORKSPACE = -3 /* workspace too small */
+};
+
+enum { SPECT_F32 = 0, SPECT_F64 = 1 };
+
+/* Rotation used by the back projection. */
+enum {
+  SPECT_BP_REFERENCE = 0, /* gather at 270-beta: spect_system_matrix.py:196 */
+  SPECT_BP_ADJOINT = 1    /* exact transpose of the forward bilinear gather */
+};
+
+/* What a back projection does with its result. */
+enum {
+  SPECT_OUT_STORE = 0,   /* out = b */
+  SPECT_OUT_ADD = 1,     /* out += b */
+  SPECT_OUT_OSEM = 2     /* out = f + f*(1/(n+delta))*(b-n), then clamp <= floor -> floor */
+};
+
+typedef struct spect_plan spect_plan;
+
+/* Geometry and transforms of one SPECT problem.
+ * Replaces SPECTObjectMeta / SPECTProjMeta (metadata/SPECT/spect_metadata.py:8-64),
+ * compute_pad_size (utils/spatial.py:36-45), the PSF layers built by
+ * SPECTPSFTransform._configure_simple_model (transforms/SPECT/psf.py:123-142)
+ * and the angle handling of SPECTSystemMatrix.forward/backward
+ * (projectors/SPECT/spect_system_matrix.py:145-154,188-198). */
+typedef struct spect_geom {
+  int32_t nx, ny, nz;      /* object shape; nx == ny (square transaxial grid) */
+  int32_t n_angles;        /* number of projections */
+  double dx, dy, dz;       /* voxel size in cm (dx scales the attenuation integral) */
+  const float *angles_deg; /* host [n_angles]: detector angle beta (float32, as proj_meta.angles) */
+  const double *angles_deg64; /* host [n_angles]: beta for float64 runs (may be NULL) */
+  /* PSF (optional): psf_kr == 0 disables it, psf_kz == 0 disables the z blur ('1D'). */
+  int32_t psf_kr, psf_kz;  /* taps per plane (odd) */
+  int32_t n_radii;         /* distinct detector radii */
+  const int32_t *radius_index; /* host [n_angles] -> [0, n_radii) */
+  const float *taps_r;     /* host [n_radii][P][psf_kr], P = nx + 2*pad */
+  const float *taps_z;     /* host [n_radii][P][psf_kz] */
+  const double *taps_r64;  /* float64 copies (may be NULL: no float64 PSF) */
+  const double *taps_z64;
+} spect_geom;
+
+const char *spect_last_error(void);
+int spect_version(void);
+
+/* Build a plan: uploads trig, linspace and PSF tables to the current device. */
+int spect_plan_create(const spect_geom *g, spect_plan **out);
+int spect_plan_destroy(spect_plan *p);
+/* Padded side P and pad of the plan. */
+int spect_plan_dims(const spect_plan *p, int32_t *padded, int32_t *pad);
+
+/* Bytes of workspace needed by forward/backward over `n` angles. */
+size_t spect_workspace_bytes(const spect_plan *p, int dtype, int n);
+
+/* Forward projection H_m f.
+ * Replaces SPECTSystemMatrix.forward (spect_system_matrix.py:120-157) with
+ * SPECTAttenuationTransform.forward (transforms/SPECT/attenuation.py:72-92)
+ * and SPECTPSFTransform.forward (transforms/SPECT/psf.py:199-214) applied in
+ * that order.
+ *   f     [nx][ny][nz]         mu [nx][ny][nz] or NULL (no attenuation)
+ *   angle_idx  device int32 [n] global angle indices
+ *   proj  [n][ny][nz] output (unpadded)                                   */
+int spect_forward(const spect_plan *p, int dtype, const void *f, const void *mu,
+                  const int32_t *angle_idx, int n, void *proj,
+                  void *workspace, size_t workspace_bytes, void *stream);
+
+/* Back projection.
+ * mode SPECT_BP_REFERENCE replaces SPECTSystemMatrix.backward
+ * (spect_system_matrix.py:159-201, PSF then attenuation then rotation by
+ * 270-beta); SPECT_BP_ADJOINT is the exact transpose of spect_forward.
+ * out_mode SPECT_OUT_OSEM fuses the OSEM update of
+ * PreconditionedGradientAscentAlgorithm.__call__ (algorithms/
+ * preconditioned_gradient_ascent.py:119-122) with OSEM's preconditioner
+ * (:319-329): out = f + (f*(1/(norm+delta)))*(b - norm), then values <= floor
+ * become floor.  f/norm are ignored for the other out modes.             */
+int spect_backward(const spect_plan *p, int dtype, int mode, int out_mode,
+                   const void *proj, const void *mu, const int32_t *angle_idx, int n,
+                   void *out, const void *f, const void *norm, double delta, double floor_value,
+                   void *workspace, size_t workspace_bytes, void *stream);
+
+/* Poisson ratio of PoissonLogLikelihood.compute_gradient
+ * (likelihoods/poisson_log_likelihood.py:34-36):
+ *   pred = hf + s (s may be NULL);  ratio = g / (pred + delta).
+ * pred may be NULL when the caller does not keep projections_predicted. */
+int spect_poisson_ratio(int dtype, const void *g, const void *hf, const void *s,
+                        void *pred, void *ratio, int64_t count, double delta, void *stream);
+
+/* Gradient of the relative difference prior
+ * (priors/nearest_neighbour.py:44-75,156-183 with EuclideanNeighbourWeight
+ * :207-220 and replicate-clamped neighbours utils/misc.py:40-57):
+ *   out = beta*scale * sum_s w_rs phi1(f_r, f_s).                        */
+int spect_rdp_gradient(int dtype, const void *f, void *out, int32_t nx, int32_t ny, int32_t nz,
+                       double dx, double dy, double dz, double beta, double gamma,
+                       double delta, double scale, void *stream);
+
+/* Linear preconditioned update of PreconditionedGradientAscentAlgorithm.__call__
+ * (preconditioned_gradient_ascent.py:119-122) for BSREM's preconditioner
+ * (:461-474):  f = f + (f*(relax/(w*norm_sum + delta)))*((b - norm) - prior);
+ * values <= floor become floor.  prior may be NULL (treated as 0).      */
+int spect_bsrem_update(int dtype, void *f, const void *b, const void *norm,
+                       const void *norm_sum, const void *prior, int64_t count,
+                       double relax, double weight, double delta, double floor_value,
+                       void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPECT_HIP_H */

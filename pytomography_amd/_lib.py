@@ -1,0 +1,131 @@
+"""ctypes binding of libspect_hip.so (declared in include/spect_hip.h).
+
+The library is the only compute path of this package: if it is missing or
+cannot be loaded, every projector call raises ``SpectLibraryError``.  There
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libspect_hip.so")
+
+SPECT_F32, SPECT_F64 = 0, 1
+BP_REFERENCE, BP_ADJOINT = 0, 1
+OUT_STORE, OUT_ADD, OUT_OSEM = 0, 1, 2
+
+EXPORTED = (
+    "spect_last_error", "spect_version", "spect_plan_create", "spect_plan_destroy",
+    "spect_plan_dims", "spect_workspace_bytes", "spect_forward", "spect_backward",
+    "spect_poisson_ratio", "spect_rdp_gradient", "spect_bsrem_update",
+)
+
+
+class SpectLibraryError(RuntimeError):
+    """libspect_hip.so is missing, failed to load, or a call returned an error."""
+
+
+class SpectGeom(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("n_angles", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("angles_deg", ctypes.POINTER(ctypes.c_float)),
+        ("angles_deg64", ctypes.POINTER(ctypes.c_double)),
+        ("psf_kr", ctypes.c_int32), ("psf_kz", ctypes.c_int32),
+        ("n_radii", ctypes.c_int32),
+        ("radius_index", ctypes.POINTER(ctypes.c_int32)),
+        ("taps_r", ctypes.POINTER(ctypes.c_float)),
+        ("taps_z", ctypes.POINTER(ctypes.c_float)),
+        ("taps_r64", ctypes.POINTER(ctypes.c_double)),
+        ("taps_z64", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+_lock = threading.Lock()
+_lib = None
+
+
+def _declare(lib):
+    vp, i32, i64, sz, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t, ctypes.c_double
+    ci = ctypes.c_int
+    lib.spect_last_error.restype = ctypes.c_char_p
+    lib.spect_last_error.argtypes = []
+    lib.spect_version.restype = ci
+    lib.spect_plan_create.restype = ci
+    lib.spect_plan_create.argtypes = [ctypes.POINTER(SpectGeom), ctypes.POINTER(vp)]
+    lib.spect_plan_destroy.restype = ci
+    lib.spect_plan_destroy.argtypes = [vp]
+    lib.spect_plan_dims.restype = ci
+    lib.spect_plan_dims.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    lib.spect_workspace_bytes.restype = sz
+    lib.spect_workspace_bytes.argtypes = [vp, ci, ci]
+    lib.spect_forward.restype = ci
+    lib.spect_forward.argtypes = [vp, ci, vp, vp, vp, ci, vp, vp, sz, vp]
+    lib.spect_backward.restype = ci
+    lib.spect_backward.argtypes = [vp, ci, ci, ci, vp, vp, vp, ci, vp, vp, vp, dbl, dbl, vp, sz, vp]
+    lib.spect_poisson_ratio.restype = ci
+    lib.spect_poisson_ratio.argtypes = [ci, vp, vp, vp, vp, vp, i64, dbl, vp]
+    lib.spect_rdp_gradient.restype = ci
+    lib.spect_rdp_gradient.argtypes = [ci, vp, vp, i32, i32, i32, dbl, dbl, dbl, dbl, dbl, dbl, dbl, vp]
+    lib.spect_bsrem_update.restype = ci
+    lib.spect_bsrem_update.argtypes = [ci, vp, vp, vp, vp, vp, i64, dbl, dbl, dbl, dbl, vp]
+
+
+def lib():
+    """The loaded library (raises SpectLibraryError when it is unavailable)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise SpectLibraryError(
+                    f"{LIB_PATH} is not built; run `python -m pytomography_amd.build` "
+                    "(the HIP kernels are the only compute path)")
+            try:
+                handle = ctypes.CDLL(LIB_PATH)
+            except OSError as exc:  # pragma: no cover - depends on the machine
+                raise SpectLibraryError(f"cannot load {LIB_PATH}: {exc}") from exc
+            _declare(handle)
+            _lib = handle
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().spect_last_error().decode(errors="replace")
+        raise SpectLibraryError(f"libspect_hip error {rc}: {msg}")
+
+
+def dtype_code(dt: torch.dtype) -> int:
+    if dt == torch.float32:
+        return SPECT_F32
+    if dt == torch.float64:
+        return SPECT_F64
+    raise SpectLibraryError(f"unsupported dtype {dt}: the kernels run in float32 or float64")
+
+
+def stream_ptr(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_device_tensor(t: torch.Tensor, name: str, dtype: torch.dtype | None = None) -> torch.Tensor:
+    """Validate a tensor handed to the kernels: on a HIP device, contiguous."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise SpectLibraryError(
+            f"{name} is on {t.device}; the SPECT kernels need a ROCm/HIP device tensor")
+    if dtype is not None and t.dtype != dtype:
+        t = t.to(dtype)
+    return t.contiguous()
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
