@@ -18,9 +18,9 @@
  *    real-valued array of the call.
  *  - Work is enqueued on `stream` (a hipStream_t, NULL = default stream);
  *    nothing synchronises the device, so calls can be captured in a hipGraph.
- *  - A plan is immutable after creation except for the attenuation cache
- *    (spect_plan_set_mu), so calls with the same plan may run on different
- *    streams as long as they do not share a workspace.
+ *  - A plan is immutable after creation, so calls with the same plan may run
+ *    on different streams as long as they do not share a workspace.
+ *  - Real-valued arrays must be 16-byte aligned (torch allocations are).
  */
 #ifndef SPECT_HIP_H
 #define SPECT_HIP_H
@@ -67,15 +67,16 @@ typedef struct spect_geom {
   int32_t n_angles;        /* number of projections */
   double dx, dy, dz;       /* voxel size in cm (dx scales the attenuation integral) */
   const float *angles_deg; /* host [n_angles]: detector angle beta (float32, as proj_meta.angles) */
-  const double *angles_deg64; /* host [n_angles]: beta for float64 runs (may be NULL) */
+  const double *angles_deg64; /* host [n_angles]: beta for float64 runs (NULL: no float64 tables) */
   /* PSF (optional): psf_kr == 0 disables it, psf_kz == 0 disables the z blur ('1D'). */
   int32_t psf_kr, psf_kz;  /* taps per plane (odd) */
   int32_t n_radii;         /* distinct detector radii */
   const int32_t *radius_index; /* host [n_angles] -> [0, n_radii) */
   const float *taps_r;     /* host [n_radii][P][psf_kr], P = nx + 2*pad */
   const float *taps_z;     /* host [n_radii][P][psf_kz] */
-  const double *taps_r64;  /* float64 copies (may be NULL: no float64 PSF) */
+  const double *taps_r64;  /* float64 copies (NULL: no float64 PSF) */
   const double *taps_z64;
+  int32_t chunk_planes;    /* planes per work chunk (0: default 16) */
 } spect_geom;
 
 const char *spect_last_error(void);
@@ -84,11 +85,21 @@ int spect_version(void);
 /* Build a plan: uploads trig, linspace and PSF tables to the current device. */
 int spect_plan_create(const spect_geom *g, spect_plan **out);
 int spect_plan_destroy(spect_plan *p);
-/* Padded side P and pad of the plan. */
-int spect_plan_dims(const spect_plan *p, int32_t *padded, int32_t *pad);
+/* Padded side P, pad, PSF half width H used by the kernels and chunk count. */
+int spect_plan_dims(const spect_plan *p, int32_t *padded, int32_t *pad, int32_t *psf_half,
+                    int32_t *chunks);
 
-/* Bytes of workspace needed by forward/backward over `n` angles. */
+/* Bytes of workspace needed by forward / backward / osem_subiteration over n angles. */
 size_t spect_workspace_bytes(const spect_plan *p, int dtype, int n);
+
+/* Attenuation table: FP64 suffix sums of mu_rot*dx at every chunk boundary
+ * for every angle -- the reverse cumsum of utils/misc.py:18 evaluated once
+ * per attenuation map (attenuation.py:87-91 recomputes it on every call).
+ * Pass it to forward/backward as att_table; NULL makes each call rebuild the
+ * rows of its own angles in the workspace. */
+size_t spect_att_table_bytes(const spect_plan *p);
+int spect_att_table_build(const spect_plan *p, int dtype, const void *mu, void *table,
+                          void *stream);
 
 /* Forward projection H_m f.
  * Replaces SPECTSystemMatrix.forward (spect_system_matrix.py:120-157) with
@@ -99,22 +110,35 @@ size_t spect_workspace_bytes(const spect_plan *p, int dtype, int n);
  *   angle_idx  device int32 [n] global angle indices
  *   proj  [n][ny][nz] output (unpadded)                                   */
 int spect_forward(const spect_plan *p, int dtype, const void *f, const void *mu,
-                  const int32_t *angle_idx, int n, void *proj,
+                  const void *att_table, const int32_t *angle_idx, int n, void *proj,
                   void *workspace, size_t workspace_bytes, void *stream);
 
 /* Back projection.
  * mode SPECT_BP_REFERENCE replaces SPECTSystemMatrix.backward
- * (spect_system_matrix.py:159-201, PSF then attenuation then rotation by
- * 270-beta); SPECT_BP_ADJOINT is the exact transpose of spect_forward.
+ * (spect_system_matrix.py:159-201: PSF (psf.py:216-231), attenuation
+ * (attenuation.py:94-115), rotation by 270-beta); SPECT_BP_ADJOINT is the
+ * exact transpose of spect_forward.
  * out_mode SPECT_OUT_OSEM fuses the OSEM update of
  * PreconditionedGradientAscentAlgorithm.__call__ (algorithms/
  * preconditioned_gradient_ascent.py:119-122) with OSEM's preconditioner
  * (:319-329): out = f + (f*(1/(norm+delta)))*(b - norm), then values <= floor
  * become floor.  f/norm are ignored for the other out modes.             */
-int spect_backward(const spect_plan *p, int dtype, int mode, int out_mode,
-                   const void *proj, const void *mu, const int32_t *angle_idx, int n,
+int spect_backward(const spect_plan *p, int dtype, int mode, int out_mode, const void *proj,
+                   const void *mu, const void *att_table, const int32_t *angle_idx, int n,
                    void *out, const void *f, const void *norm, double delta, double floor_value,
                    void *workspace, size_t workspace_bytes, void *stream);
+
+/* One OSEM sub-iteration in place on f (the body of the subset loop of
+ * preconditioned_gradient_ascent.py:103-122 for OSEM with
+ * PoissonLogLikelihood.compute_gradient, poisson_log_likelihood.py:32-36):
+ *   pred = H_m f + s;  r = g / (pred + delta);  b = B_m r (reference back
+ *   projection);  f = f + (f*(1/(norm+delta)))*(b - norm);  f[f<=floor] = floor.
+ * g, s (NULL: no additive term) and pred are [n][ny][nz]; pred may be NULL. */
+int spect_osem_subiteration(const spect_plan *p, int dtype, void *f, const void *mu,
+                            const void *att_table, const int32_t *angle_idx, int n, const void *g,
+                            const void *s, const void *norm, void *pred, double delta,
+                            double floor_value, void *workspace, size_t workspace_bytes,
+                            void *stream);
 
 /* Poisson ratio of PoissonLogLikelihood.compute_gradient
  * (likelihoods/poisson_log_likelihood.py:34-36):

@@ -21,7 +21,8 @@ OUT_STORE, OUT_ADD, OUT_OSEM = 0, 1, 2
 
 EXPORTED = (
     "spect_last_error", "spect_version", "spect_plan_create", "spect_plan_destroy",
-    "spect_plan_dims", "spect_workspace_bytes", "spect_forward", "spect_backward",
+    "spect_plan_dims", "spect_workspace_bytes", "spect_att_table_bytes", "spect_att_table_build",
+    "spect_forward", "spect_backward", "spect_osem_subiteration",
     "spect_poisson_ratio", "spect_rdp_gradient", "spect_bsrem_update",
 )
 
@@ -44,6 +45,7 @@ class SpectGeom(ctypes.Structure):
         ("taps_z", ctypes.POINTER(ctypes.c_float)),
         ("taps_r64", ctypes.POINTER(ctypes.c_double)),
         ("taps_z64", ctypes.POINTER(ctypes.c_double)),
+        ("chunk_planes", ctypes.c_int32),
     ]
 
 
@@ -62,13 +64,19 @@ def _declare(lib):
     lib.spect_plan_destroy.restype = ci
     lib.spect_plan_destroy.argtypes = [vp]
     lib.spect_plan_dims.restype = ci
-    lib.spect_plan_dims.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    lib.spect_plan_dims.argtypes = [vp] + [ctypes.POINTER(i32)] * 4
     lib.spect_workspace_bytes.restype = sz
     lib.spect_workspace_bytes.argtypes = [vp, ci, ci]
+    lib.spect_att_table_bytes.restype = sz
+    lib.spect_att_table_bytes.argtypes = [vp]
+    lib.spect_att_table_build.restype = ci
+    lib.spect_att_table_build.argtypes = [vp, ci, vp, vp, vp]
     lib.spect_forward.restype = ci
-    lib.spect_forward.argtypes = [vp, ci, vp, vp, vp, ci, vp, vp, sz, vp]
+    lib.spect_forward.argtypes = [vp, ci, vp, vp, vp, vp, ci, vp, vp, sz, vp]
     lib.spect_backward.restype = ci
-    lib.spect_backward.argtypes = [vp, ci, ci, ci, vp, vp, vp, ci, vp, vp, vp, dbl, dbl, vp, sz, vp]
+    lib.spect_backward.argtypes = [vp, ci, ci, ci, vp, vp, vp, vp, ci, vp, vp, vp, dbl, dbl, vp, sz, vp]
+    lib.spect_osem_subiteration.restype = ci
+    lib.spect_osem_subiteration.argtypes = [vp, ci, vp, vp, vp, vp, ci, vp, vp, vp, vp, dbl, dbl, vp, sz, vp]
     lib.spect_poisson_ratio.restype = ci
     lib.spect_poisson_ratio.argtypes = [ci, vp, vp, vp, vp, vp, i64, dbl, vp]
     lib.spect_rdp_gradient.restype = ci
@@ -116,7 +124,8 @@ def stream_ptr(device: torch.device) -> int:
 
 
 def require_device_tensor(t: torch.Tensor, name: str, dtype: torch.dtype | None = None) -> torch.Tensor:
-    """Validate a tensor handed to the kernels: on a HIP device, contiguous."""
+    """Validate a tensor handed to the kernels: on a HIP device, contiguous,
+    16-byte aligned (a misaligned view is copied)."""
     if not isinstance(t, torch.Tensor):
         raise TypeError(f"{name} must be a torch.Tensor")
     if t.device.type != "cuda":
@@ -124,7 +133,10 @@ def require_device_tensor(t: torch.Tensor, name: str, dtype: torch.dtype | None 
             f"{name} is on {t.device}; the SPECT kernels need a ROCm/HIP device tensor")
     if dtype is not None and t.dtype != dtype:
         t = t.to(dtype)
-    return t.contiguous()
+    t = t.contiguous()
+    if t.data_ptr() % 16:
+        t = t.clone()
+    return t
 
 
 def ptr(t: torch.Tensor | None) -> int | None:

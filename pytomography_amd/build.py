@@ -2,8 +2,9 @@
 
     python -m pytomography_amd.build        (or __graft_entry__.build())
 
-The library is a plain C-ABI shared object (no torch, no Python headers),
-loaded by ``pytomography_amd._lib`` through ctypes.
+Each translation unit under csrc/ is compiled in parallel, then linked into a
+plain C-ABI shared object (no torch, no Python headers) that
+``pytomography_amd._lib`` loads through ctypes.
 """
 from __future__ import annotations
 
@@ -11,14 +12,16 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "spect_kernels.hip")
+CSRC = os.path.join(HERE, "csrc")
+SOURCES = ["forward.hip", "backward.hip", "misc.hip", "capi.hip"]
 OUT = os.path.join(HERE, "libspect_hip.so")
 ARCH = os.environ.get("SPECT_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "-fPIC",
     f"--offload-arch={ARCH}",
     # packed FP32 instructions stay off on this target
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -34,12 +37,25 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = True) -> str:
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, SRC, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, OUT)
+    cc = hipcc()
+    with tempfile.TemporaryDirectory(prefix="spect_build_") as tmp:
+        procs, objs = [], []
+        for src in SOURCES:
+            obj = os.path.join(tmp, src.replace(".hip", ".o"))
+            cmd = [cc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((src, subprocess.Popen(cmd)))
+            objs.append(obj)
+        failed = [src for src, p in procs if p.wait() != 0]
+        if failed:
+            raise RuntimeError(f"hipcc failed for {failed}")
+        tmp_out = OUT + ".tmp"
+        link = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp_out]
+        if verbose:
+            print(" ".join(link), flush=True)
+        subprocess.run(link, check=True)
+        os.replace(tmp_out, OUT)
     return OUT
 
 

@@ -1,0 +1,365 @@
+// backward.hip -- back projection kernels.
+//
+// SPECTSystemMatrix.backward (spect_system_matrix.py:159-201): per angle the
+// padded projection is smeared along x (:180,191), blurred by the PSF
+// (psf.py:216-231), multiplied by the attenuation factor (attenuation.py:
+// 94-115), rotated back with kornia rotate(270 - beta) (:196) and summed over
+// the angles, then cropped (:200).
+//
+// bp_field_kernel: one block = (angle, tile of kTJ rotated-frame columns,
+//   chunk of L planes); writes W(i,j,z) = a(i,j,z) * (G_i * p)(j,z).  The y
+//   blur reads a window of the projection that each thread keeps in
+//   registers for all planes; the z blur uses a register window with an LDS
+//   halo; a is the FP64 reverse-cumsum attenuation as in the forward kernel.
+//   Planes whose tile lies outside the (1.5-pixel widened) footprint are
+//   skipped: the gathers below never read them.
+// bp_ref_kernel: W gathered at theta_b = 270 - beta, summed over the angles
+//   in registers, cropped, then stored / added / used for the OSEM update.
+// bp_adj_kernel: the exact transpose of the forward's bilinear gather,
+//   written as a gather over the <= 16 rotated-frame candidates of each voxel
+//   (race-free and deterministic).
+#include <type_traits>
+
+#include "../../include/spect_hip.h"
+#include "launch.hpp"
+
+namespace spect {
+namespace {
+
+template <typename R, int RZ, int H>
+__global__ __launch_bounds__(512) void bp_field_kernel(Geo<R> g, Tables<R> tb,
+                                                       const R *__restrict__ proj,
+                                                       const R *__restrict__ mu,
+                                                       const double *__restrict__ att,
+                                                       int att_global,
+                                                       const int32_t *__restrict__ angle_idx,
+                                                       R *__restrict__ W) {
+  constexpr int K = 2 * H + 1;
+  constexpr int HALF_ROWS = kTJ / 2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ Src<R> src[2][kTJ];
+  const int al = blockIdx.x, jt = blockIdx.y, c = blockIdx.z;
+  const int ang = angle_idx[al];
+  if ((unsigned)ang >= (unsigned)g.A) return;
+  const int tid = threadIdx.x;
+  const int Nz = g.Nz;
+  const int runs = (Nz + RZ - 1) / RZ;
+  const int ldu = runs * RZ + 1;
+  R *U = reinterpret_cast<R *>(smem);  // [kTJ][ldu] after the y blur
+  const int j0 = g.je0 + jt * kTJ;
+  const int tj = min(kTJ, g.je1 - j0);
+  const int n_jext = g.je1 - g.je0;
+  const int nout = g.jo1 - g.jo0;
+  // phase A ownership: one z, half of the tile rows (wave-uniform half)
+  const int nzc = (Nz + 63) / 64 * 64;
+  const int half = tid / nzc, za = tid - half * nzc;
+  const bool aact = half < 2 && za < Nz;
+  // phase B ownership: column jl, z run [z0, z0 + RZ)
+  const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
+  const bool bact = run < runs;
+  const bool mine = bact && jl < tj;
+  const auto cs = tb.trig[ang];
+  const bool has_mu = mu != nullptr;
+
+  // projection window for the y blur: tile rows [half*16 - H, half*16 + 16 + H)
+  R pw[HALF_ROWS + 2 * H];
+#pragma unroll
+  for (int m = 0; m < HALF_ROWS + 2 * H; ++m) {
+    pw[m] = R(0);
+    if (aact) {
+      const int j = j0 + half * HALF_ROWS - H + m;
+      if (j >= g.jo0 && j < g.jo1) pw[m] = proj[((size_t)al * nout + (j - g.jo0)) * Nz + za];
+    }
+  }
+  double S[RZ];
+#pragma unroll
+  for (int m = 0; m < RZ; ++m) {
+    S[m] = 0.0;
+    if (has_mu && mine && z0 + m < Nz) {
+      const int rowa = att_global ? ang : al;
+      S[m] = att[(((size_t)rowa * g.C + c) * n_jext + (j0 - g.je0 + jl)) * Nz + z0 + m];
+    }
+  }
+  const int rix = tb.radius_index[ang];
+  const int i_lo = c * g.L, i_hi = min(g.P, i_lo + g.L);
+  int buf = 0;
+  for (int i = i_hi - 1; i >= i_lo; --i) {
+    int inside = 0;
+    if (tid < tj) {
+      R ix, iy;
+      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], tb.lin[i], g.half, ix, iy);
+      src[buf][tid] = make_src<R>(ix, iy, g.pad);
+      inside = in_footprint<R>(ix, iy, g);
+    }
+    if (!__syncthreads_or(inside)) continue;
+    R bz[RZ];
+    if constexpr (H > 0) {
+      // ---- A: y blur of the projection with this plane's taps -> U
+      if (aact) {
+        const R *wy = tb.taps_r + ((size_t)rix * g.P + i) * K;
+#pragma unroll
+        for (int r = 0; r < HALF_ROWS; ++r) {
+          R u = R(0);
+#pragma unroll
+          for (int t = 0; t < K; ++t) u += wy[t] * pw[r + t];
+          U[(size_t)(half * HALF_ROWS + r) * ldu + za] = u;
+        }
+      }
+      __syncthreads();
+      // ---- B: z blur of the own run
+      if (bact) {
+        const R *wz = tb.taps_z + ((size_t)rix * g.P + i) * K;
+        const R *ur = U + (size_t)jl * ldu;
+        R win[RZ + 2 * H];
+#pragma unroll
+        for (int m = 0; m < RZ + 2 * H; ++m) {
+          const int z = z0 - H + m;
+          win[m] = (z >= 0 && z < Nz) ? ur[z] : R(0);
+        }
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) {
+          R b = R(0);
+#pragma unroll
+          for (int t = 0; t < K; ++t) b += wz[t] * win[m + t];
+          bz[m] = b;
+        }
+      }
+    } else {
+      // no PSF: the smeared projection itself
+      if (bact) {
+        const int j = j0 + jl;
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) {
+          const int z = z0 + m;
+          bz[m] = (j >= g.jo0 && j < g.jo1 && z < Nz) ? proj[((size_t)al * nout + (j - g.jo0)) * Nz + z]
+                                                       : R(0);
+        }
+      }
+    }
+    // ---- attenuation and store
+    if (mine) {
+      if (has_mu) {
+        const Taps4<R> t = taps4<R>(src[buf][jl], g.N, Nz);
+        R x[RZ];
+        gather_run<R, RZ>(mu, t, z0, g.N, Nz, x);
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) {
+          const R xm = x[m] * g.dx;
+          S[m] += (double)xm;
+          bz[m] = bz[m] * att_factor<R>(S[m], xm);
+        }
+      }
+      R *dst = W + (((size_t)al * g.P + i) * n_jext + (j0 - g.je0 + jl)) * Nz + z0;
+      if (z0 + RZ <= Nz && (Nz % 4) == 0) {
+        if constexpr (sizeof(R) == 4) {
+#pragma unroll
+          for (int m = 0; m < RZ; m += 4)
+            *reinterpret_cast<float4 *>(dst + m) = make_float4(bz[m], bz[m + 1], bz[m + 2], bz[m + 3]);
+        } else {
+#pragma unroll
+          for (int m = 0; m < RZ; m += 2)
+            *reinterpret_cast<double2 *>(dst + m) = make_double2(bz[m], bz[m + 1]);
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < RZ; ++m)
+          if (z0 + m < Nz) dst[m] = bz[m];
+      }
+    }
+    buf ^= 1;
+  }
+}
+
+// Output stage of a back projection for one voxel value b.
+template <typename R>
+__device__ __forceinline__ void bp_store(int out_mode, R *__restrict__ out, size_t o, R b,
+                                         const R *__restrict__ f, const R *__restrict__ nrm,
+                                         R delta, R floor_value) {
+#pragma clang fp contract(off)
+  if (out_mode == SPECT_OUT_STORE) {
+    out[o] = b;
+  } else if (out_mode == SPECT_OUT_ADD) {
+    out[o] = out[o] + b;
+  } else {
+    // preconditioned_gradient_ascent.py:121-122 with OSEM's factor (:329)
+    const R n = nrm[o];
+    const R fo = f[o];
+    const R pre = fo * (R(1) / (n + delta));
+    const R grad = b - n;
+    R v = fo + pre * (grad - R(0));
+    if (v <= floor_value) v = floor_value;
+    out[o] = v;
+  }
+}
+
+// One wave per object column (x, y); lanes run along z.
+template <typename R>
+__global__ __launch_bounds__(256) void bp_ref_kernel(Geo<R> g, Tables<R> tb,
+                                                     const R *__restrict__ W,
+                                                     const int32_t *__restrict__ angle_idx,
+                                                     int n_call, int out_mode, R *__restrict__ out,
+                                                     const R *__restrict__ f,
+                                                     const R *__restrict__ nrm, R delta,
+                                                     R floor_value) {
+  const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int Nz = g.Nz, N = g.N;
+  const int zblocks = (Nz + 63) >> 6;
+  const int col = wave / zblocks;
+  if (col >= N * N) return;
+  const int z = (wave - col * zblocks) * 64 + lane;
+  if (z >= Nz) return;
+  const int x = col / N, y = col - x * N;
+  const int X = x + g.pad, Y = y + g.pad;
+  const int n_jext = g.je1 - g.je0;
+  const R xn = tb.lin[Y], yn = tb.lin[X];
+  R acc = R(0);
+  for (int al = 0; al < n_call; ++al) {
+    const int ang = angle_idx[al];
+    if ((unsigned)ang >= (unsigned)g.A) continue;
+    const auto cs = tb.trig[ang];
+    R ix, iy;
+    rot_coord<R>(cs.x, -cs.y, xn, yn, g.half, ix, iy);  // theta_b = -theta_f
+    const R fx = floor(ix), fy = floor(iy);
+    const int c0 = int(fx), r0 = int(fy);
+    const R tx = ix - fx, ty = iy - fy;
+    const R ex = R(1) - tx, ny = R(1) - ty;
+    const R *Wa = W + (size_t)al * g.P * n_jext * Nz;
+    auto tap = [&](int r, int cc) -> R {
+      if ((unsigned)r >= (unsigned)g.P || cc < g.je0 || cc >= g.je1) return R(0);
+      return Wa[((size_t)r * n_jext + (cc - g.je0)) * Nz + z];
+    };
+    R v = tap(r0, c0) * (ny * ex);
+    v = v + tap(r0, c0 + 1) * (ny * tx);
+    v = v + tap(r0 + 1, c0) * (ty * ex);
+    v = v + tap(r0 + 1, c0 + 1) * (ty * tx);
+    acc = acc + v;
+  }
+  bp_store<R>(out_mode, out, ((size_t)x * N + y) * Nz + z, acc, f, nrm, delta, floor_value);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void bp_adj_kernel(Geo<R> g, Tables<R> tb,
+                                                     const R *__restrict__ W,
+                                                     const int32_t *__restrict__ angle_idx,
+                                                     int n_call, int out_mode, R *__restrict__ out,
+                                                     const R *__restrict__ f,
+                                                     const R *__restrict__ nrm, R delta,
+                                                     R floor_value) {
+  const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int Nz = g.Nz, N = g.N;
+  const int zblocks = (Nz + 63) >> 6;
+  const int col = wave / zblocks;
+  if (col >= N * N) return;
+  const int z = (wave - col * zblocks) * 64 + lane;
+  if (z >= Nz) return;
+  const int x = col / N, y = col - x * N;
+  const int X = x + g.pad, Y = y + g.pad;
+  const int n_jext = g.je1 - g.je0;
+  R acc = R(0);
+  for (int al = 0; al < n_call; ++al) {
+    const int ang = angle_idx[al];
+    if ((unsigned)ang >= (unsigned)g.A) continue;
+    const auto cs = tb.trig[ang];
+    // rotated-frame position whose forward sample lands on (row X, col Y)
+    R jc, ic;
+    rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, jc, ic);
+    const int ib = int(floor(ic - R(1.5))), jb = int(floor(jc - R(1.5)));
+    const R *Wa = W + (size_t)al * g.P * n_jext * Nz;
+    R v = R(0);
+    for (int di = 0; di < 4; ++di) {
+      const int i = ib + di;
+      if ((unsigned)i >= (unsigned)g.P) continue;
+      for (int dj = 0; dj < 4; ++dj) {
+        const int j = jb + dj;
+        if (j < g.je0 || j >= g.je1) continue;
+        R ix, iy;
+        rot_coord<R>(cs.x, cs.y, tb.lin[j], tb.lin[i], g.half, ix, iy);
+        const R fx = floor(ix), fy = floor(iy);
+        const int c0 = int(fx), r0 = int(fy);
+        const R tx = ix - fx, ty = iy - fy;
+        R wr, wc;
+        if (r0 == X) wr = R(1) - ty;
+        else if (r0 + 1 == X) wr = ty;
+        else continue;
+        if (c0 == Y) wc = R(1) - tx;
+        else if (c0 + 1 == Y) wc = tx;
+        else continue;
+        v = v + Wa[((size_t)i * n_jext + (j - g.je0)) * Nz + z] * (wr * wc);
+      }
+    }
+    acc = acc + v;
+  }
+  bp_store<R>(out_mode, out, ((size_t)x * N + y) * Nz + z, acc, f, nrm, delta, floor_value);
+}
+
+template <typename R, int H>
+hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
+                          const double *att, bool att_global, const int32_t *idx, int n, R *W,
+                          hipStream_t st) {
+  constexpr int RZ = RunLen<R>::value;
+  const int bt = tile_block_threads(g.Nz, RZ);
+  const int runs = (g.Nz + RZ - 1) / RZ;
+  const size_t smem = (size_t)kTJ * (runs * RZ + 1) * sizeof(R);
+  auto kern = bp_field_kernel<R, RZ, H>;
+  if (smem > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+  }
+  dim3 grid(n, g.ntiles, g.C);
+  kern<<<grid, bt, smem, st>>>(g, tb, proj, mu, att, att_global ? 1 : 0, idx, W);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+template <typename R>
+hipError_t launch_bp_field(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
+                           const double *att, bool att_global, const int32_t *idx, int n, R *W,
+                           hipStream_t st) {
+  switch (g.H) {
+    case 0: return launch_field_h<R, 0>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    case 2: return launch_field_h<R, 2>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    case 4: return launch_field_h<R, 4>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    case 6: return launch_field_h<R, 6>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    case 8: return launch_field_h<R, 8>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    case 10: return launch_field_h<R, 10>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    case 12: return launch_field_h<R, 12>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    case 14: return launch_field_h<R, 14>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    case 16: return launch_field_h<R, 16>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    case 18: return launch_field_h<R, 18>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    case 20: return launch_field_h<R, 20>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename R>
+hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
+                            const int32_t *idx, int n, int mode, int out_mode, R *out, const R *f,
+                            const R *nrm, R delta, R floor_value, hipStream_t st) {
+  const int zblocks = (g.Nz + 63) / 64;
+  const size_t waves = (size_t)g.N * g.N * zblocks;
+  const unsigned blocks = (unsigned)((waves * 64 + 255) / 256);
+  if (mode == SPECT_BP_REFERENCE)
+    bp_ref_kernel<R><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm, delta,
+                                             floor_value);
+  else
+    bp_adj_kernel<R><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm, delta,
+                                             floor_value);
+  return hipGetLastError();
+}
+
+#define SPECT_INST(R)                                                                          \
+  template hipError_t launch_bp_field<R>(const Geo<R> &, const Tables<R> &, const R *,         \
+                                         const R *, const double *, bool, const int32_t *, int, \
+                                         R *, hipStream_t);                                    \
+  template hipError_t launch_bp_gather<R>(const Geo<R> &, const Tables<R> &, const R *,        \
+                                          const int32_t *, int, int, int, R *, const R *,      \
+                                          const R *, R, R, hipStream_t);
+SPECT_INST(float)
+SPECT_INST(double)
+#undef SPECT_INST
+
+}  // namespace spect

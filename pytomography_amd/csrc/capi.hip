@@ -1,0 +1,491 @@
+// capi.hip -- plan management and the extern "C" entry points of libspect_hip.so
+// (declared in include/spect_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spect_hip.h"
+#include "launch.hpp"
+
+#define SPECT_VERSION 2
+
+using namespace spect;
+
+struct spect_plan {
+  int N, Nz, A, P, pad;
+  double dx, dy, dz;
+  int H;                 // PSF half width used by the kernels (0: none)
+  int L, C;              // planes per chunk, chunk count
+  int n_radii;
+  bool has64;
+  int device;
+  float2 *trig32 = nullptr;
+  double2 *trig64 = nullptr;
+  float *lin32 = nullptr;
+  double *lin64 = nullptr;
+  int32_t *radius_index = nullptr;
+  int32_t *all_angles = nullptr;  // 0..A-1
+  float *taps_r32 = nullptr, *taps_z32 = nullptr;
+  double *taps_r64 = nullptr, *taps_z64 = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return fail(SPECT_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                          \
+  do {                                                         \
+    hipError_t e_ = (expr);                                    \
+    if (e_ != hipSuccess) return hip_fail(e_, #expr);          \
+  } while (0)
+
+int host_pad_size(int width) { return (int)std::ceil((std::sqrt(2.0) * width - width) / 2.0); }
+
+// torch.linspace(-1, 1, P) as ATen's CPU range kernel computes it: a step in
+// the output dtype, ascending from -1 for the first half, descending from +1.
+template <typename R> std::vector<R> host_linspace(int P) {
+  std::vector<R> v(P);
+  if (P == 1) {
+    v[0] = R(-1);
+    return v;
+  }
+  const R step = (R(1) - R(-1)) / R(P - 1);
+  const int halfway = P / 2;
+  for (int k = 0; k < P; ++k) v[k] = k < halfway ? R(-1) + step * R(k) : R(1) - step * R(P - k - 1);
+  return v;
+}
+
+template <typename T> int upload(T **dst, const T *src, size_t count) {
+  HIP_TRY(hipMalloc((void **)dst, count * sizeof(T)));
+  HIP_TRY(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return SPECT_OK;
+}
+
+// Re-centre per-plane taps of width k into width K = 2H+1 (zero taps outside):
+// a zero tap adds an exact 0 to the correlation.
+template <typename T>
+std::vector<T> widen_taps(const T *src, int rows, int k, int K, bool unit_if_null) {
+  std::vector<T> out((size_t)rows * K, T(0));
+  const int off = (K - k) / 2;
+  for (int r = 0; r < rows; ++r) {
+    if (src) {
+      for (int t = 0; t < k; ++t) out[(size_t)r * K + off + t] = src[(size_t)r * k + t];
+    } else if (unit_if_null) {
+      out[(size_t)r * K + K / 2] = T(1);
+    }
+  }
+  return out;
+}
+
+template <typename R> Tables<R> tables(const spect_plan *p);
+template <> Tables<float> tables<float>(const spect_plan *p) {
+  return {p->trig32, p->lin32, p->radius_index, p->taps_r32, p->taps_z32};
+}
+template <> Tables<double> tables<double>(const spect_plan *p) {
+  return {p->trig64, p->lin64, p->radius_index, p->taps_r64, p->taps_z64};
+}
+
+template <typename R> Geo<R> geo(const spect_plan *p) {
+  Geo<R> g;
+  g.N = p->N;
+  g.Nz = p->Nz;
+  g.P = p->P;
+  g.pad = p->pad;
+  g.A = p->A;
+  g.H = p->H;
+  g.jo0 = p->pad;
+  g.jo1 = p->pad + p->N;
+  g.je0 = std::max(0, g.jo0 - p->H);
+  g.je1 = std::min(p->P, g.jo1 + p->H);
+  g.L = p->L;
+  g.C = p->C;
+  g.ntiles = (g.je1 - g.je0 + kTJ - 1) / kTJ;
+  g.half = R(p->P - 1) / R(2);
+  g.dx = R(p->dx);
+  g.lo = R(p->pad) - R(2);
+  g.hi = R(p->pad + p->N) + R(1);
+  return g;
+}
+
+struct Ws {
+  size_t att_off, part_off, w_off, ratio_off, total;
+};
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+Ws ws_layout(const spect_plan *p, int dtype, int n) {
+  const size_t rs = dtype == SPECT_F64 ? 8 : 4;
+  const Geo<float> g = geo<float>(p);
+  const size_t n_jext = g.je1 - g.je0;
+  const size_t JO = kTJ + 2 * p->H;
+  const size_t att = align256((size_t)n * p->C * n_jext * p->Nz * sizeof(double));
+  const size_t part = align256((size_t)p->C * n * g.ntiles * JO * p->Nz * rs);
+  const size_t w = align256((size_t)n * p->P * n_jext * p->Nz * rs);
+  const size_t ratio = align256((size_t)n * p->N * p->Nz * rs);
+  Ws l;
+  l.att_off = 0;
+  l.part_off = att;
+  l.w_off = att;  // forward partials are consumed before the field is written
+  l.ratio_off = att + std::max(part, w);
+  l.total = l.ratio_off + ratio;
+  return l;
+}
+
+bool aligned16(const void *ptr) { return ((uintptr_t)ptr & 15) == 0; }
+
+int check_call(const spect_plan *p, int dtype, int n, const void *ws, size_t ws_bytes) {
+  if (!p) return fail(SPECT_E_ARG, "null plan");
+  if (dtype != SPECT_F32 && dtype != SPECT_F64) return fail(SPECT_E_ARG, "dtype must be F32 or F64");
+  if (dtype == SPECT_F64 && !p->has64) return fail(SPECT_E_ARG, "plan has no float64 tables");
+  if (n <= 0 || n > p->A) return fail(SPECT_E_ARG, "angle count out of range");
+  const size_t need = ws_layout(p, dtype, n).total;
+  if (!ws || ws_bytes < need)
+    return fail(SPECT_E_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  if (!aligned16(ws)) return fail(SPECT_E_ARG, "workspace must be 16-byte aligned");
+  return SPECT_OK;
+}
+
+int check_arrays(std::initializer_list<const void *> arrays) {
+  for (const void *a : arrays)
+    if (a && !aligned16(a)) return fail(SPECT_E_ARG, "arrays must be 16-byte aligned");
+  return SPECT_OK;
+}
+
+// Attenuation rows for the call: the caller's table (indexed by global angle)
+// or rows built now in the workspace (indexed by position in angle_idx).
+template <typename R>
+int att_rows(const spect_plan *p, const Geo<R> &g, const R *mu, const void *table,
+             const int32_t *idx, int n, unsigned char *ws, const Ws &l, hipStream_t st,
+             const double **att, bool *global) {
+  *att = nullptr;
+  *global = false;
+  if (!mu) return SPECT_OK;
+  if (table) {
+    *att = reinterpret_cast<const double *>(table);
+    *global = true;
+    return SPECT_OK;
+  }
+  double *rows = reinterpret_cast<double *>(ws + l.att_off);
+  HIP_TRY(launch_att_table<R>(g, tables<R>(p), mu, idx, n, rows, st));
+  *att = rows;
+  return SPECT_OK;
+}
+
+template <typename R>
+int forward_impl(const spect_plan *p, const R *f, const R *mu, const void *table,
+                 const int32_t *idx, int n, R *proj, const R *gm, const R *s, R *pred, R *ratio,
+                 R delta, unsigned char *ws, hipStream_t st) {
+  const Geo<R> g = geo<R>(p);
+  const Ws l = ws_layout(p, sizeof(R) == 8 ? SPECT_F64 : SPECT_F32, n);
+  const double *att;
+  bool global;
+  int rc = att_rows<R>(p, g, mu, table, idx, n, ws, l, st, &att, &global);
+  if (rc) return rc;
+  R *part = reinterpret_cast<R *>(ws + l.part_off);
+  HIP_TRY(launch_forward<R>(g, tables<R>(p), f, mu, att, global, idx, n, part, st));
+  HIP_TRY(launch_forward_reduce<R>(g, part, n, proj, gm, s, pred, ratio, delta, st));
+  return SPECT_OK;
+}
+
+template <typename R>
+int backward_impl(const spect_plan *p, int mode, int out_mode, const R *proj, const R *mu,
+                  const void *table, const int32_t *idx, int n, R *out, const R *f, const R *nrm,
+                  double delta, double floor_value, unsigned char *ws, hipStream_t st) {
+  const Geo<R> g = geo<R>(p);
+  const Ws l = ws_layout(p, sizeof(R) == 8 ? SPECT_F64 : SPECT_F32, n);
+  const double *att;
+  bool global;
+  int rc = att_rows<R>(p, g, mu, table, idx, n, ws, l, st, &att, &global);
+  if (rc) return rc;
+  R *W = reinterpret_cast<R *>(ws + l.w_off);
+  HIP_TRY(launch_bp_field<R>(g, tables<R>(p), proj, mu, att, global, idx, n, W, st));
+  HIP_TRY(launch_bp_gather<R>(g, tables<R>(p), W, idx, n, mode, out_mode, out, f, nrm, R(delta),
+                              R(floor_value), st));
+  return SPECT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *spect_last_error(void) { return g_err.c_str(); }
+
+int spect_version(void) { return SPECT_VERSION; }
+
+int spect_plan_create(const spect_geom *g, spect_plan **out) {
+  if (!g || !out) return fail(SPECT_E_ARG, "null argument");
+  *out = nullptr;
+  if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0) return fail(SPECT_E_ARG, "object shape must be positive");
+  if (g->nx != g->ny) return fail(SPECT_E_ARG, "nx must equal ny (square transaxial grid)");
+  if (g->nz > 256) return fail(SPECT_E_ARG, "nz > 256 is not supported by the tile kernels");
+  if (g->n_angles <= 0 || !g->angles_deg) return fail(SPECT_E_ARG, "angles required");
+  if (!(g->dx > 0) || !(g->dz > 0)) return fail(SPECT_E_ARG, "voxel sizes must be positive");
+  if (g->psf_kr < 0 || g->psf_kz < 0 || (g->psf_kr > 0 && g->psf_kr % 2 == 0) ||
+      (g->psf_kz > 0 && g->psf_kz % 2 == 0))
+    return fail(SPECT_E_ARG, "PSF tap counts must be odd");
+  if (g->psf_kr > 0 && (g->n_radii <= 0 || !g->radius_index || !g->taps_r ||
+                        (g->psf_kz > 0 && !g->taps_z)))
+    return fail(SPECT_E_ARG, "PSF tables missing");
+  const int hmax = std::max(g->psf_kr, g->psf_kz) / 2;
+  int H = g->psf_kr > 0 ? (hmax + 1) / 2 * 2 : 0;  // even half widths are compiled
+  if (g->psf_kr > 0 && H == 0) H = 2;
+  if (H > 20) return fail(SPECT_E_ARG, "PSF wider than 41 taps is not supported");
+
+  auto *p = new spect_plan();
+  p->N = g->nx;
+  p->Nz = g->nz;
+  p->A = g->n_angles;
+  p->pad = host_pad_size(g->ny);
+  p->P = g->nx + 2 * p->pad;
+  p->dx = g->dx;
+  p->dy = g->dy;
+  p->dz = g->dz;
+  p->H = H;
+  p->L = g->chunk_planes > 0 ? g->chunk_planes : 16;
+  p->C = (p->P + p->L - 1) / p->L;
+  p->n_radii = g->psf_kr > 0 ? g->n_radii : 1;
+  p->has64 = g->angles_deg64 != nullptr && (g->psf_kr == 0 || g->taps_r64 != nullptr);
+  (void)hipGetDevice(&p->device);
+  auto cleanup = [&](int rc) {
+    spect_plan_destroy(p);
+    return rc;
+  };
+  // theta_f = -(270 - beta) in float32; radians as kornia's deg2rad in float32
+  std::vector<float2> t32(p->A);
+  std::vector<double2> t64(p->A);
+  const float pi32 = (float)M_PI;
+  for (int a = 0; a < p->A; ++a) {
+    const float th = -(270.0f - g->angles_deg[a]);
+    const float rad = (th * pi32) / 180.0f;
+    t32[a] = make_float2((float)std::cos((double)rad), (float)std::sin((double)rad));
+    const double b64 = g->angles_deg64 ? g->angles_deg64[a] : (double)g->angles_deg[a];
+    const double r64 = -(270.0 - b64) * M_PI / 180.0;
+    t64[a] = make_double2(std::cos(r64), std::sin(r64));
+  }
+  std::vector<int32_t> ridx(p->A, 0), all(p->A);
+  for (int a = 0; a < p->A; ++a) {
+    all[a] = a;
+    if (g->psf_kr > 0) {
+      if (g->radius_index[a] < 0 || g->radius_index[a] >= p->n_radii) {
+        delete p;
+        return fail(SPECT_E_ARG, "radius_index out of range");
+      }
+      ridx[a] = g->radius_index[a];
+    }
+  }
+  int rc;
+  if ((rc = upload(&p->trig32, t32.data(), t32.size()))) return cleanup(rc);
+  if ((rc = upload(&p->trig64, t64.data(), t64.size()))) return cleanup(rc);
+  auto l32 = host_linspace<float>(p->P);
+  auto l64 = host_linspace<double>(p->P);
+  if ((rc = upload(&p->lin32, l32.data(), l32.size()))) return cleanup(rc);
+  if ((rc = upload(&p->lin64, l64.data(), l64.size()))) return cleanup(rc);
+  if ((rc = upload(&p->radius_index, ridx.data(), ridx.size()))) return cleanup(rc);
+  if ((rc = upload(&p->all_angles, all.data(), all.size()))) return cleanup(rc);
+  if (H > 0) {
+    const int K = 2 * H + 1;
+    const int rows = p->n_radii * p->P;
+    auto r32 = widen_taps<float>(g->taps_r, rows, g->psf_kr, K, false);
+    auto z32 = widen_taps<float>(g->psf_kz > 0 ? g->taps_z : nullptr, rows, g->psf_kz, K, true);
+    if ((rc = upload(&p->taps_r32, r32.data(), r32.size()))) return cleanup(rc);
+    if ((rc = upload(&p->taps_z32, z32.data(), z32.size()))) return cleanup(rc);
+    if (p->has64) {
+      auto r64 = widen_taps<double>(g->taps_r64, rows, g->psf_kr, K, false);
+      auto z64 = widen_taps<double>(g->psf_kz > 0 ? g->taps_z64 : nullptr, rows, g->psf_kz, K, true);
+      if ((rc = upload(&p->taps_r64, r64.data(), r64.size()))) return cleanup(rc);
+      if ((rc = upload(&p->taps_z64, z64.data(), z64.size()))) return cleanup(rc);
+    }
+  }
+  *out = p;
+  return SPECT_OK;
+}
+
+int spect_plan_destroy(spect_plan *p) {
+  if (!p) return SPECT_OK;
+  (void)hipFree(p->trig32);
+  (void)hipFree(p->trig64);
+  (void)hipFree(p->lin32);
+  (void)hipFree(p->lin64);
+  (void)hipFree(p->radius_index);
+  (void)hipFree(p->all_angles);
+  (void)hipFree(p->taps_r32);
+  (void)hipFree(p->taps_z32);
+  (void)hipFree(p->taps_r64);
+  (void)hipFree(p->taps_z64);
+  delete p;
+  return SPECT_OK;
+}
+
+int spect_plan_dims(const spect_plan *p, int32_t *padded, int32_t *pad, int32_t *psf_half,
+                    int32_t *chunks) {
+  if (!p) return fail(SPECT_E_ARG, "null plan");
+  if (padded) *padded = p->P;
+  if (pad) *pad = p->pad;
+  if (psf_half) *psf_half = p->H;
+  if (chunks) *chunks = p->C;
+  return SPECT_OK;
+}
+
+size_t spect_workspace_bytes(const spect_plan *p, int dtype, int n) {
+  if (!p || n <= 0) return 0;
+  return ws_layout(p, dtype, n).total;
+}
+
+size_t spect_att_table_bytes(const spect_plan *p) {
+  if (!p) return 0;
+  const Geo<float> g = geo<float>(p);
+  return (size_t)p->A * p->C * (g.je1 - g.je0) * p->Nz * sizeof(double);
+}
+
+int spect_att_table_build(const spect_plan *p, int dtype, const void *mu, void *table,
+                          void *stream) {
+  if (!p || !mu || !table) return fail(SPECT_E_ARG, "null argument");
+  if (int rc = check_arrays({mu, table})) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == SPECT_F32) {
+    HIP_TRY(launch_att_table<float>(geo<float>(p), tables<float>(p), (const float *)mu,
+                                    p->all_angles, p->A, (double *)table, st));
+  } else if (dtype == SPECT_F64) {
+    if (!p->has64) return fail(SPECT_E_ARG, "plan has no float64 tables");
+    HIP_TRY(launch_att_table<double>(geo<double>(p), tables<double>(p), (const double *)mu,
+                                     p->all_angles, p->A, (double *)table, st));
+  } else {
+    return fail(SPECT_E_ARG, "bad dtype");
+  }
+  return SPECT_OK;
+}
+
+int spect_forward(const spect_plan *p, int dtype, const void *f, const void *mu,
+                  const void *att_table, const int32_t *angle_idx, int n, void *proj,
+                  void *workspace, size_t workspace_bytes, void *stream) {
+  if (int rc = check_call(p, dtype, n, workspace, workspace_bytes)) return rc;
+  if (!f || !proj || !angle_idx) return fail(SPECT_E_ARG, "null array argument");
+  if (int rc = check_arrays({f, mu, att_table, proj})) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  auto *ws = reinterpret_cast<unsigned char *>(workspace);
+  if (dtype == SPECT_F32)
+    return forward_impl<float>(p, (const float *)f, (const float *)mu, att_table, angle_idx, n,
+                               (float *)proj, nullptr, nullptr, nullptr, nullptr, 0.f, ws, st);
+  return forward_impl<double>(p, (const double *)f, (const double *)mu, att_table, angle_idx, n,
+                              (double *)proj, nullptr, nullptr, nullptr, nullptr, 0.0, ws, st);
+}
+
+int spect_backward(const spect_plan *p, int dtype, int mode, int out_mode, const void *proj,
+                   const void *mu, const void *att_table, const int32_t *angle_idx, int n,
+                   void *out, const void *f, const void *norm, double delta, double floor_value,
+                   void *workspace, size_t workspace_bytes, void *stream) {
+  if (int rc = check_call(p, dtype, n, workspace, workspace_bytes)) return rc;
+  if (!proj || !out || !angle_idx) return fail(SPECT_E_ARG, "null array argument");
+  if (mode != SPECT_BP_REFERENCE && mode != SPECT_BP_ADJOINT) return fail(SPECT_E_ARG, "bad mode");
+  if (out_mode < SPECT_OUT_STORE || out_mode > SPECT_OUT_OSEM) return fail(SPECT_E_ARG, "bad out_mode");
+  if (out_mode == SPECT_OUT_OSEM && (!f || !norm)) return fail(SPECT_E_ARG, "OSEM update needs f and norm");
+  if (int rc = check_arrays({proj, mu, att_table, out, f, norm})) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  auto *ws = reinterpret_cast<unsigned char *>(workspace);
+  if (dtype == SPECT_F32)
+    return backward_impl<float>(p, mode, out_mode, (const float *)proj, (const float *)mu,
+                                att_table, angle_idx, n, (float *)out, (const float *)f,
+                                (const float *)norm, delta, floor_value, ws, st);
+  return backward_impl<double>(p, mode, out_mode, (const double *)proj, (const double *)mu,
+                               att_table, angle_idx, n, (double *)out, (const double *)f,
+                               (const double *)norm, delta, floor_value, ws, st);
+}
+
+int spect_osem_subiteration(const spect_plan *p, int dtype, void *f, const void *mu,
+                            const void *att_table, const int32_t *angle_idx, int n, const void *g,
+                            const void *s, const void *norm, void *pred, double delta,
+                            double floor_value, void *workspace, size_t workspace_bytes,
+                            void *stream) {
+  if (int rc = check_call(p, dtype, n, workspace, workspace_bytes)) return rc;
+  if (!f || !g || !norm || !angle_idx) return fail(SPECT_E_ARG, "null array argument");
+  if (int rc = check_arrays({f, mu, att_table, g, s, norm, pred})) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  auto *ws = reinterpret_cast<unsigned char *>(workspace);
+  const Ws l = ws_layout(p, dtype, n);
+  if (dtype == SPECT_F32) {
+    float *ratio = reinterpret_cast<float *>(ws + l.ratio_off);
+    int rc = forward_impl<float>(p, (const float *)f, (const float *)mu, att_table, angle_idx, n,
+                                 nullptr, (const float *)g, (const float *)s, (float *)pred, ratio,
+                                 (float)delta, ws, st);
+    if (rc) return rc;
+    return backward_impl<float>(p, SPECT_BP_REFERENCE, SPECT_OUT_OSEM, ratio, (const float *)mu,
+                                att_table, angle_idx, n, (float *)f, (const float *)f,
+                                (const float *)norm, delta, floor_value, ws, st);
+  }
+  double *ratio = reinterpret_cast<double *>(ws + l.ratio_off);
+  int rc = forward_impl<double>(p, (const double *)f, (const double *)mu, att_table, angle_idx, n,
+                                nullptr, (const double *)g, (const double *)s, (double *)pred,
+                                ratio, delta, ws, st);
+  if (rc) return rc;
+  return backward_impl<double>(p, SPECT_BP_REFERENCE, SPECT_OUT_OSEM, ratio, (const double *)mu,
+                               att_table, angle_idx, n, (double *)f, (const double *)f,
+                               (const double *)norm, delta, floor_value, ws, st);
+}
+
+int spect_poisson_ratio(int dtype, const void *g, const void *hf, const void *s, void *pred,
+                        void *ratio, int64_t count, double delta, void *stream) {
+  if (!g || !hf || !ratio || count < 0) return fail(SPECT_E_ARG, "bad ratio arguments");
+  if (count == 0) return SPECT_OK;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_ratio<float>((const float *)g, (const float *)hf, (const float *)s,
+                                (float *)pred, (float *)ratio, count, (float)delta, st));
+  else if (dtype == SPECT_F64)
+    HIP_TRY(launch_ratio<double>((const double *)g, (const double *)hf, (const double *)s,
+                                 (double *)pred, (double *)ratio, count, delta, st));
+  else
+    return fail(SPECT_E_ARG, "bad dtype");
+  return SPECT_OK;
+}
+
+int spect_rdp_gradient(int dtype, const void *f, void *out, int32_t nx, int32_t ny, int32_t nz,
+                       double dx, double dy, double dz, double beta, double gamma, double delta,
+                       double scale, void *stream) {
+  if (!f || !out || nx <= 0 || ny <= 0 || nz <= 0) return fail(SPECT_E_ARG, "bad prior arguments");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const double bs = beta * scale;
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_rdp<float>((const float *)f, (float *)out, nx, ny, nz, dx, dy, dz, (float)bs,
+                              (float)gamma, (float)delta, st));
+  else if (dtype == SPECT_F64)
+    HIP_TRY(launch_rdp<double>((const double *)f, (double *)out, nx, ny, nz, dx, dy, dz, bs, gamma,
+                               delta, st));
+  else
+    return fail(SPECT_E_ARG, "bad dtype");
+  return SPECT_OK;
+}
+
+int spect_bsrem_update(int dtype, void *f, const void *b, const void *norm, const void *norm_sum,
+                       const void *prior, int64_t count, double relax, double weight, double delta,
+                       double floor_value, void *stream) {
+  if (!f || !b || !norm || !norm_sum || count < 0) return fail(SPECT_E_ARG, "bad update arguments");
+  if (count == 0) return SPECT_OK;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_bsrem_update<float>((float *)f, (const float *)b, (const float *)norm,
+                                       (const float *)norm_sum, (const float *)prior, count,
+                                       (float)relax, (float)weight, (float)delta,
+                                       (float)floor_value, st));
+  else if (dtype == SPECT_F64)
+    HIP_TRY(launch_bsrem_update<double>((double *)f, (const double *)b, (const double *)norm,
+                                        (const double *)norm_sum, (const double *)prior, count,
+                                        relax, weight, delta, floor_value, st));
+  else
+    return fail(SPECT_E_ARG, "bad dtype");
+  return SPECT_OK;
+}
+
+}  // extern "C"

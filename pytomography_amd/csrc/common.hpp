@@ -1,0 +1,186 @@
+// common.hpp -- device helpers and launch descriptors shared by the SPECT kernels.
+//
+// Frames.  The object f[x][y][z] (z fastest) sits in a zero-padded P x P
+// transaxial grid, P = N + 2*pad (utils/spatial.py:36-81).  For detector angle
+// beta the "rotated frame" V(i, j, z) is the object resampled by kornia's
+// rotate(theta_f = beta - 270) (spect_system_matrix.py:150,
+// transforms/shared/spatial.py:53): rotated-frame pixel (row i, column j)
+// samples the padded object at
+//     col = (c*xn_j - s*yn_i + 1) * (P-1)/2,  row = (s*xn_j + c*yn_i + 1) * (P-1)/2
+// with xn = linspace(-1, 1, P) and (c, s) = (cos, sin)(theta_f), bilinear
+// weights and zeros outside (grid_sample, align_corners=True).  The detector
+// is at +i; a projection is the sum over i (spect_system_matrix.py:154).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace spect {
+
+constexpr int kTJ = 32;  // rotated-frame columns per tile (forward and back projection)
+
+template <typename R> struct V2;
+template <> struct V2<float> { using T = float2; };
+template <> struct V2<double> { using T = double2; };
+
+// z values per thread in the gather / z-blur phases (registers hold them).
+template <typename R> struct RunLen;
+template <> struct RunLen<float> { static constexpr int value = 16; };
+template <> struct RunLen<double> { static constexpr int value = 8; };
+
+// Device tables of a plan in one precision.
+template <typename R> struct Tables {
+  const typename V2<R>::T *trig;  // [A] (cos, sin) of theta_f = beta - 270
+  const R *lin;                   // [P] linspace(-1, 1, P)
+  const int32_t *radius_index;    // [A] -> PSF table row block
+  const R *taps_r;                // [n_radii][P][K] taps along r (y), K = 2H+1
+  const R *taps_z;                // [n_radii][P][K] taps along z (a unit tap when '1D')
+};
+
+// Geometry of one call.
+template <typename R> struct Geo {
+  int N, Nz, P, pad, A;
+  int H;          // PSF half width (0: no PSF)
+  int je0, je1;   // rotated-frame columns that can be non-zero after the y blur
+  int jo0, jo1;   // detector columns kept by unpad_proj
+  int L, C;       // planes per chunk, chunks
+  int ntiles;     // ceil((je1 - je0) / kTJ)
+  R half;         // (P - 1) / 2
+  R dx;           // attenuation step (cm)
+  R lo, hi;       // footprint test: a sample matters iff lo < col, row < hi
+};
+
+// Bilinear source of one rotated-frame pixel: rows/cols in unpadded object
+// coordinates (may be out of range) plus fractional parts.
+template <typename R> struct Src {
+  int r0, c0;
+  R tx, ty;
+};
+
+// affine_grid with theta = [[c, -s, 0], [s, c, 0]] followed by grid_sample's
+// unnormalisation (g + 1) * (size - 1) / 2: column ix and row iy (padded px).
+template <typename R>
+__device__ __forceinline__ void rot_coord(R c, R s, R xn, R yn, R half, R &ix, R &iy) {
+  const R gx = c * xn - s * yn;
+  const R gy = s * xn + c * yn;
+  ix = (gx + R(1)) * half;
+  iy = (gy + R(1)) * half;
+}
+
+template <typename R>
+__device__ __forceinline__ Src<R> make_src(R ix, R iy, int pad) {
+  const R fx = floor(ix), fy = floor(iy);
+  Src<R> s;
+  s.c0 = int(fx) - pad;
+  s.r0 = int(fy) - pad;
+  s.tx = ix - fx;
+  s.ty = iy - fy;
+  return s;
+}
+
+// Bilinear weights in grid_sample's CPU order: nw=(1-ty)(1-tx), ne=(1-ty)tx,
+// sw=ty(1-tx), se=ty*tx; out-of-object taps weigh zero.
+template <typename R> struct Taps4 {
+  size_t base;        // element offset of (r0, c0, 0)
+  R wnw, wne, wsw, wse;
+  bool nw, ne, sw, se;
+};
+
+template <typename R>
+__device__ __forceinline__ Taps4<R> taps4(const Src<R> &s, int N, int Nz) {
+  Taps4<R> t;
+  const R ex = R(1) - s.tx, ny = R(1) - s.ty;
+  const bool r0 = (unsigned)s.r0 < (unsigned)N, r1 = (unsigned)(s.r0 + 1) < (unsigned)N;
+  const bool c0 = (unsigned)s.c0 < (unsigned)N, c1 = (unsigned)(s.c0 + 1) < (unsigned)N;
+  t.nw = r0 && c0;
+  t.ne = r0 && c1;
+  t.sw = r1 && c0;
+  t.se = r1 && c1;
+  t.wnw = ny * ex;
+  t.wne = ny * s.tx;
+  t.wsw = s.ty * ex;
+  t.wse = s.ty * s.tx;
+  // Offset of (r0, c0); it wraps for negative rows/columns, which is fine
+  // because it is only dereferenced (with unsigned wrap-around arithmetic)
+  // through a tap that lies inside the object.
+  t.base = (size_t)((long long)s.r0 * N + s.c0) * Nz;
+  return t;
+}
+
+template <typename R>
+__device__ __forceinline__ R interp(const R *__restrict__ vol, const Taps4<R> &t, int z, int N,
+                                    int Nz) {
+  const R vnw = t.nw ? vol[t.base + z] : R(0);
+  const R vne = t.ne ? vol[t.base + Nz + z] : R(0);
+  const R vsw = t.sw ? vol[t.base + (size_t)N * Nz + z] : R(0);
+  const R vse = t.se ? vol[t.base + (size_t)N * Nz + Nz + z] : R(0);
+  R acc = vnw * t.wnw;
+  acc = acc + vne * t.wne;
+  acc = acc + vsw * t.wsw;
+  acc = acc + vse * t.wse;
+  return acc;
+}
+
+// Gather RZ consecutive z of one rotated-frame pixel (bilinear), vectorised
+// when the run is aligned and complete.
+template <typename R, int RZ>
+__device__ __forceinline__ void gather_run(const R *__restrict__ vol, const Taps4<R> &t, int z0,
+                                           int N, int Nz, R (&out)[RZ]) {
+  if (z0 + RZ <= Nz && (Nz % 4) == 0) {
+#pragma unroll
+    for (int m = 0; m < RZ; ++m) out[m] = R(0);
+    const size_t offs[4] = {t.base, t.base + Nz, t.base + (size_t)N * Nz, t.base + (size_t)N * Nz + Nz};
+    const bool ok[4] = {t.nw, t.ne, t.sw, t.se};
+    const R w[4] = {t.wnw, t.wne, t.wsw, t.wse};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      R v[RZ];
+      if (ok[q]) {
+        const R *p = vol + offs[q] + z0;
+        if constexpr (sizeof(R) == 4) {
+#pragma unroll
+          for (int m = 0; m < RZ; m += 4) {
+            const float4 x = *reinterpret_cast<const float4 *>(p + m);
+            v[m] = x.x; v[m + 1] = x.y; v[m + 2] = x.z; v[m + 3] = x.w;
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < RZ; m += 2) {
+            const double2 x = *reinterpret_cast<const double2 *>(p + m);
+            v[m] = x.x; v[m + 1] = x.y;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) v[m] = R(0);
+      }
+      // ((nw + ne) + sw) + se, as interp()
+#pragma unroll
+      for (int m = 0; m < RZ; ++m) out[m] = q == 0 ? v[m] * w[0] : out[m] + v[m] * w[q];
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < RZ; ++m) out[m] = (z0 + m < Nz) ? interp<R>(vol, t, z0 + m, N, Nz) : R(0);
+  }
+}
+
+__device__ __forceinline__ float att_exp(float v) { return expf(v); }
+__device__ __forceinline__ double att_exp(double v) { return exp(v); }
+
+// Reverse-cumsum attenuation factor (utils/misc.py:18, attenuation.py:21):
+// x = mu_rot*dx in R; S (FP64) already includes x; a = exp(-(R(S) - x/2)).
+template <typename R>
+__device__ __forceinline__ R att_factor(double S, R x) {
+  const R c = R(S);
+  const R r = c - x / R(2);
+  return att_exp(-r);
+}
+
+// Does rotated-frame pixel (i, j) sample inside the footprint square?
+template <typename R>
+__device__ __forceinline__ bool in_footprint(R ix, R iy, const Geo<R> &g) {
+  return ix > g.lo && ix < g.hi && iy > g.lo && iy < g.hi;
+}
+
+}  // namespace spect

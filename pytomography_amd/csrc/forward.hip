@@ -1,0 +1,360 @@
+// forward.hip -- attenuation table and forward projection kernels.
+//
+// SPECTSystemMatrix.forward (spect_system_matrix.py:120-157) with
+// SPECTAttenuationTransform.forward (attenuation.py:72-92) and
+// SPECTPSFTransform.forward (psf.py:199-214, Seperable1DBlurNet psf.py:27-41).
+//
+// One block = (angle, tile of kTJ rotated-frame columns, chunk of L planes).
+// The block walks its planes from the detector side (high i) down.  Per plane:
+//   A  gather: V(j,z) = a(i,j,z) * f_rot(i,j,z) for the tile's columns; each
+//      thread owns one column and a run of RZ z values in registers; the
+//      attenuation a comes from an FP64 running sum of mu_rot*dx that starts at
+//      the chunk's suffix sum (att table) -- the reverse cumsum of
+//      utils/misc.py:18 in the order a sequential scan would add.
+//   B  z blur with the plane's taps: register window (own run + halo read from
+//      LDS).
+//   C  y blur written as a scatter: each thread owns one z and half of the
+//      tile's kTJ + 2H output columns and accumulates, in registers across all
+//      planes, sum_t w[t] * Vz(j + t - H); the ray sum is that accumulation.
+// Planes where no column of the tile samples the object are skipped.  Each
+// block writes its kTJ + 2H output columns; a reduce kernel adds chunks and
+// overlapping tiles in a fixed order (deterministic, no atomics).
+#include <type_traits>
+
+#include "launch.hpp"
+
+namespace spect {
+
+int tile_block_threads(int Nz, int run_len) {
+  const int runs = (Nz + run_len - 1) / run_len;
+  const int nzc = (Nz + 63) / 64 * 64;
+  int t = kTJ * runs;
+  if (2 * nzc > t) t = 2 * nzc;
+  return (t + 63) / 64 * 64;
+}
+
+namespace {
+
+template <typename R, int RZ>
+__global__ __launch_bounds__(512) void att_table_kernel(Geo<R> g, Tables<R> tb,
+                                                        const R *__restrict__ mu,
+                                                        const int32_t *__restrict__ angle_idx,
+                                                        double *__restrict__ table) {
+  __shared__ Src<R> src[kTJ];
+  const int al = blockIdx.x, jt = blockIdx.y;
+  const int ang = angle_idx[al];
+  if ((unsigned)ang >= (unsigned)g.A) return;
+  const int tid = threadIdx.x;
+  const int j0 = g.je0 + jt * kTJ;
+  const int tj = min(kTJ, g.je1 - j0);
+  const int runs = (g.Nz + RZ - 1) / RZ;
+  const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
+  const bool mine = run < runs && jl < tj;
+  const int n_jext = g.je1 - g.je0;
+  const auto cs = tb.trig[ang];
+  double S[RZ];
+#pragma unroll
+  for (int m = 0; m < RZ; ++m) S[m] = 0.0;
+  auto row = [&](int c, int m) -> size_t {
+    return (((size_t)al * g.C + c) * n_jext + (j0 - g.je0 + jl)) * g.Nz + z0 + m;
+  };
+  if (mine) {
+#pragma unroll
+    for (int m = 0; m < RZ; ++m)
+      if (z0 + m < g.Nz) table[row(g.C - 1, m)] = 0.0;
+  }
+  for (int i = g.P - 1; i >= 0; --i) {
+    int inside = 0;
+    if (tid < tj) {
+      R ix, iy;
+      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], tb.lin[i], g.half, ix, iy);
+      src[tid] = make_src<R>(ix, iy, g.pad);
+      inside = in_footprint<R>(ix, iy, g);
+    }
+    if (__syncthreads_or(inside)) {
+      if (mine) {
+        const Taps4<R> t = taps4<R>(src[jl], g.N, g.Nz);
+        R x[RZ];
+        gather_run<R, RZ>(mu, t, z0, g.N, g.Nz, x);
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) S[m] += (double)(x[m] * g.dx);
+      }
+      __syncthreads();  // src is rewritten by the next plane
+    }
+    if (mine && i > 0 && i % g.L == 0) {
+      const int c = i / g.L - 1;
+#pragma unroll
+      for (int m = 0; m < RZ; ++m)
+        if (z0 + m < g.Nz) table[row(c, m)] = S[m];
+    }
+  }
+}
+
+template <typename R, int RZ, int H>
+__global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const R *__restrict__ f,
+                                                  const R *__restrict__ mu,
+                                                  const double *__restrict__ att, int att_global,
+                                                  const int32_t *__restrict__ angle_idx,
+                                                  R *__restrict__ partial, int n_call) {
+  constexpr int K = 2 * H + 1;
+  constexpr int JO = kTJ + 2 * H;      // output columns touched by the tile
+  constexpr int JH = (JO + 1) / 2;     // per half
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ Src<R> src[kTJ];
+  const int al = blockIdx.x, jt = blockIdx.y, c = blockIdx.z;
+  const int ang = angle_idx[al];
+  if ((unsigned)ang >= (unsigned)g.A) return;
+  const int tid = threadIdx.x;
+  const int Nz = g.Nz;
+  const int runs = (Nz + RZ - 1) / RZ;
+  const int ldv = runs * RZ + 1;  // odd row stride: conflict-free column reads
+  R *V = reinterpret_cast<R *>(smem);
+  R *Vz = V + (size_t)kTJ * ldv;
+  const int j0 = g.je0 + jt * kTJ;
+  const int tj = min(kTJ, g.je1 - j0);
+  const int n_jext = g.je1 - g.je0;
+  // phase A/B ownership: column jl, z run [z0, z0 + RZ)
+  const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
+  const bool ab = run < runs;
+  const bool mine = ab && jl < tj;
+  // phase C ownership: one z, half of the output columns (wave-uniform half)
+  const int nzc = (Nz + 63) / 64 * 64;
+  const int half = tid / nzc, zc = tid - half * nzc;
+  const bool cact = half < 2 && zc < Nz;
+  const auto cs = tb.trig[ang];
+  const bool has_mu = mu != nullptr;
+
+  double S[RZ];
+#pragma unroll
+  for (int m = 0; m < RZ; ++m) {
+    S[m] = 0.0;
+    if (has_mu && mine && z0 + m < Nz) {
+      const int rowa = att_global ? ang : al;
+      S[m] = att[(((size_t)rowa * g.C + c) * n_jext + (j0 - g.je0 + jl)) * Nz + z0 + m];
+    }
+  }
+  R acc[JH];
+#pragma unroll
+  for (int q = 0; q < JH; ++q) acc[q] = R(0);
+
+  const int rix = tb.radius_index[ang];
+  const int i_lo = c * g.L, i_hi = min(g.P, i_lo + g.L);
+  for (int i = i_hi - 1; i >= i_lo; --i) {
+    int inside = 0;
+    if (tid < tj) {
+      R ix, iy;
+      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], tb.lin[i], g.half, ix, iy);
+      src[tid] = make_src<R>(ix, iy, g.pad);
+      inside = in_footprint<R>(ix, iy, g);
+    }
+    if (!__syncthreads_or(inside)) continue;
+
+    // ---- A: V = a * f_rot for the own column / z run
+    R own[RZ];
+    if (mine) {
+      const Taps4<R> t = taps4<R>(src[jl], g.N, Nz);
+      gather_run<R, RZ>(f, t, z0, g.N, Nz, own);
+      if (has_mu) {
+        R x[RZ];
+        gather_run<R, RZ>(mu, t, z0, g.N, Nz, x);
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) {
+          const R xm = x[m] * g.dx;
+          S[m] += (double)xm;
+          own[m] = own[m] * att_factor<R>(S[m], xm);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < RZ; ++m) own[m] = R(0);
+    }
+    if constexpr (H == 0) {
+      if (ab) {
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) Vz[(size_t)jl * ldv + z0 + m] = own[m];
+      }
+      __syncthreads();
+      if (cact) {
+        // no PSF: the ray sum of column jl is the plain sum over planes
+#pragma unroll
+        for (int q = 0; q < JH; ++q) {
+          const int jo = half * JH + q;
+          if (jo < JO) acc[q] += Vz[(size_t)jo * ldv + zc];
+        }
+      }
+      __syncthreads();
+    } else {
+      if (ab) {
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) V[(size_t)jl * ldv + z0 + m] = own[m];
+      }
+      __syncthreads();
+      // ---- B: z blur (correlation, zero padded) of the own run
+      const R *wz = tb.taps_z + ((size_t)rix * g.P + i) * K;
+      if (ab) {
+        R win[RZ + 2 * H];
+        const R *vr = V + (size_t)jl * ldv;
+#pragma unroll
+        for (int m = 0; m < H; ++m) {
+          const int zl = z0 - H + m, zr = z0 + RZ + m;
+          win[m] = zl >= 0 ? vr[zl] : R(0);
+          win[H + RZ + m] = zr < Nz ? vr[zr] : R(0);
+        }
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) win[H + m] = own[m];
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) {
+          R u = R(0);
+#pragma unroll
+          for (int t = 0; t < K; ++t) u += wz[t] * win[m + t];
+          Vz[(size_t)jl * ldv + z0 + m] = u;
+        }
+      }
+      __syncthreads();
+      // ---- C: y blur as a scatter into the tile's output columns
+      if (cact) {
+        const R *wy = tb.taps_r + ((size_t)rix * g.P + i) * K;
+        auto scatter = [&](auto half_c) {
+          constexpr int HC = decltype(half_c)::value;
+          constexpr int jlo = HC * JH;
+          constexpr int jhi = (jlo + JH < JO) ? jlo + JH : JO;
+          constexpr int jv0 = (jlo - 2 * H > 0) ? jlo - 2 * H : 0;
+          constexpr int jv1 = jhi < kTJ ? jhi : kTJ;
+#pragma unroll
+          for (int jv = jv0; jv < jv1; ++jv) {
+            const R v = Vz[(size_t)jv * ldv + zc];
+#pragma unroll
+            for (int q = 0; q < JH; ++q) {
+              const int jo = jlo + q;
+              const int t = jv - jo + 2 * H;  // out[jo] += w[t] * in[jo + t - 2H] (tile-local)
+              if (jo < jhi && t >= 0 && t < K) acc[q] += wy[t] * v;
+            }
+          }
+        };
+        if (half == 0)
+          scatter(std::integral_constant<int, 0>{});
+        else
+          scatter(std::integral_constant<int, 1>{});
+      }
+    }
+  }
+  if (cact) {
+    R *dst = partial + ((((size_t)c * n_call + al) * g.ntiles + jt) * JO) * Nz;
+#pragma unroll
+    for (int q = 0; q < JH; ++q) {
+      const int jo = half * JH + q;
+      if (jo < JO) dst[(size_t)jo * Nz + zc] = acc[q];
+    }
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void fwd_reduce_kernel(Geo<R> g, const R *__restrict__ partial,
+                                                         int n, R *__restrict__ proj,
+                                                         const R *__restrict__ gm,
+                                                         const R *__restrict__ s,
+                                                         R *__restrict__ pred, R *__restrict__ ratio,
+                                                         R delta) {
+  const int nout = g.jo1 - g.jo0;
+  const size_t count = (size_t)n * nout * g.Nz;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= count) return;
+  const int z = idx % g.Nz;
+  const int jx = (idx / g.Nz) % nout;
+  const int al = idx / ((size_t)g.Nz * nout);
+  const int JO = kTJ + 2 * g.H;
+  const int j = g.jo0 + jx;
+  const int rel = j - g.je0 + g.H;  // column within tile t: rel - t*kTJ
+  int t_hi = rel / kTJ;
+  if (t_hi > g.ntiles - 1) t_hi = g.ntiles - 1;
+  int t_lo = (rel - JO) < 0 ? 0 : (rel - JO) / kTJ + 1;
+  R acc = R(0);
+  for (int c = 0; c < g.C; ++c) {
+    for (int t = t_lo; t <= t_hi; ++t) {
+      const int jo = rel - t * kTJ;
+      acc += partial[((((size_t)c * n + al) * g.ntiles + t) * JO + jo) * g.Nz + z];
+    }
+  }
+  if (proj) proj[idx] = acc;
+  if (gm) {
+    R p = acc;
+    if (s) p = p + s[idx];
+    if (pred) pred[idx] = p;
+    ratio[idx] = gm[idx] / (p + delta);
+  }
+}
+
+template <typename R, int H>
+hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *mu,
+                        const double *att, bool att_global, const int32_t *idx, int n, R *partial,
+                        hipStream_t st) {
+  constexpr int RZ = RunLen<R>::value;
+  const int bt = tile_block_threads(g.Nz, RZ);
+  const int runs = (g.Nz + RZ - 1) / RZ;
+  const size_t smem = 2 * (size_t)kTJ * (runs * RZ + 1) * sizeof(R);
+  auto kern = fwd_kernel<R, RZ, H>;
+  if (smem > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+  }
+  dim3 grid(n, g.ntiles, g.C);
+  kern<<<grid, bt, smem, st>>>(g, tb, f, mu, att, att_global ? 1 : 0, idx, partial, n);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+template <typename R>
+hipError_t launch_att_table(const Geo<R> &g, const Tables<R> &tb, const R *mu,
+                            const int32_t *angle_idx, int n, double *table, hipStream_t st) {
+  constexpr int RZ = RunLen<R>::value;
+  const int runs = (g.Nz + RZ - 1) / RZ;
+  const int bt = (kTJ * runs + 63) / 64 * 64;
+  dim3 grid(n, g.ntiles);
+  att_table_kernel<R, RZ><<<grid, bt, 0, st>>>(g, tb, mu, angle_idx, table);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_forward(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *mu,
+                          const double *att, bool att_global, const int32_t *idx, int n,
+                          R *partial, hipStream_t st) {
+  switch (g.H) {
+    case 0: return launch_fwd_h<R, 0>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    case 2: return launch_fwd_h<R, 2>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    case 4: return launch_fwd_h<R, 4>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    case 6: return launch_fwd_h<R, 6>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    case 8: return launch_fwd_h<R, 8>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    case 10: return launch_fwd_h<R, 10>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    case 12: return launch_fwd_h<R, 12>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    case 14: return launch_fwd_h<R, 14>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    case 16: return launch_fwd_h<R, 16>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    case 18: return launch_fwd_h<R, 18>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    case 20: return launch_fwd_h<R, 20>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename R>
+hipError_t launch_forward_reduce(const Geo<R> &g, const R *partial, int n, R *proj, const R *gm,
+                                 const R *s, R *pred, R *ratio, R delta, hipStream_t st) {
+  const size_t count = (size_t)n * (g.jo1 - g.jo0) * g.Nz;
+  fwd_reduce_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(g, partial, n, proj, gm, s,
+                                                                         pred, ratio, delta);
+  return hipGetLastError();
+}
+
+#define SPECT_INST(R)                                                                          \
+  template hipError_t launch_att_table<R>(const Geo<R> &, const Tables<R> &, const R *,        \
+                                          const int32_t *, int, double *, hipStream_t);        \
+  template hipError_t launch_forward<R>(const Geo<R> &, const Tables<R> &, const R *,          \
+                                        const R *, const double *, bool, const int32_t *, int, \
+                                        R *, hipStream_t);                                     \
+  template hipError_t launch_forward_reduce<R>(const Geo<R> &, const R *, int, R *, const R *, \
+                                               const R *, R *, R *, R, hipStream_t);
+SPECT_INST(float)
+SPECT_INST(double)
+#undef SPECT_INST
+
+}  // namespace spect

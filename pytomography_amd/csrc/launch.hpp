@@ -1,0 +1,61 @@
+// launch.hpp -- host-side launchers of the SPECT kernels (one per translation unit).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace spect {
+
+// Threads per block of the tile kernels for a z extent Nz.
+int tile_block_threads(int Nz, int run_len);
+
+// Suffix sums of mu_rot*dx (FP64) at every chunk boundary, for the angles in
+// angle_idx; row of angle a is (by_global ? angle_idx[a] : a).
+// table layout: [rows][C][je1-je0][Nz] doubles.
+template <typename R>
+hipError_t launch_att_table(const Geo<R> &g, const Tables<R> &tb, const R *mu,
+                            const int32_t *angle_idx, int n, double *table, hipStream_t st);
+
+// Forward projection partials: partial[C][n][ntiles][kTJ + 2H][Nz].
+template <typename R>
+hipError_t launch_forward(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *mu,
+                          const double *att, bool att_global, const int32_t *angle_idx, int n,
+                          R *partial, hipStream_t st);
+
+// proj[n][jo1-jo0][Nz] = sum of the partials; if g_meas != nullptr also
+// pred = proj + s (s may be null) and ratio = g_meas / (pred + delta).
+template <typename R>
+hipError_t launch_forward_reduce(const Geo<R> &g, const R *partial, int n, R *proj,
+                                 const R *g_meas, const R *s, R *pred, R *ratio, R delta,
+                                 hipStream_t st);
+
+// Back projection field W[n][P][je1-je0][Nz] = a * (G_i * p).
+template <typename R>
+hipError_t launch_bp_field(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
+                           const double *att, bool att_global, const int32_t *angle_idx, int n,
+                           R *W, hipStream_t st);
+
+// Gather W back into the object (mode: 0 reference rotation, 1 exact adjoint)
+// with out_mode 0 store, 1 add, 2 OSEM update.
+template <typename R>
+hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
+                            const int32_t *angle_idx, int n, int mode, int out_mode, R *out,
+                            const R *f, const R *nrm, R delta, R floor_value, hipStream_t st);
+
+// Elementwise and stencil kernels (misc.hip).
+template <typename R>
+hipError_t launch_ratio(const R *g, const R *hf, const R *s, R *pred, R *ratio, int64_t count,
+                        R delta, hipStream_t st);
+template <typename R>
+hipError_t launch_rdp(const R *f, R *out, int nx, int ny, int nz, double dx, double dy, double dz,
+                      R beta_scale, R gamma, R delta, hipStream_t st);
+template <typename R>
+hipError_t launch_bsrem_update(R *f, const R *b, const R *nrm, const R *nsum, const R *prior,
+                               int64_t count, R relax, R weight, R delta, R floor_value,
+                               hipStream_t st);
+
+}  // namespace spect

@@ -1,0 +1,117 @@
+// misc.hip -- elementwise / stencil kernels of the likelihood, prior and updates.
+#include "../../include/spect_hip.h"
+#include "launch.hpp"
+
+namespace spect {
+namespace {
+
+// likelihoods/poisson_log_likelihood.py:34-36
+template <typename R>
+__global__ __launch_bounds__(256) void ratio_kernel(const R *__restrict__ g,
+                                                    const R *__restrict__ hf,
+                                                    const R *__restrict__ s, R *__restrict__ pred,
+                                                    R *__restrict__ ratio, int64_t count, R delta) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= count) return;
+  R p = hf[idx];
+  if (s) p = p + s[idx];
+  if (pred) pred[idx] = p;
+  ratio[idx] = g[idx] / (p + delta);
+}
+
+// RelativeDifferencePrior.phi1 (priors/nearest_neighbour.py:176-177) summed
+// over the 26 replicate-clamped neighbours (utils/misc.py:40-57) with
+// EuclideanNeighbourWeight (:207-220), in the reference's loop order.
+template <typename R>
+__global__ __launch_bounds__(256) void rdp_kernel(const R *__restrict__ f, R *__restrict__ out,
+                                                  int nx, int ny, int nz, double dx, double dy,
+                                                  double dz, R beta_scale, R gamma, R delta) {
+#pragma clang fp contract(off)
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t count = (int64_t)nx * ny * nz;
+  if (idx >= count) return;
+  const int z = idx % nz;
+  const int y = (idx / nz) % ny;
+  const int x = idx / ((int64_t)nz * ny);
+  const R fr = f[idx];
+  R acc = R(0);
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j)
+      for (int k = -1; k <= 1; ++k) {
+        if (i == 0 && j == 0 && k == 0) continue;
+        const double w = dx / sqrt((dx * i) * (dx * i) + (dy * j) * (dy * j) + (dz * k) * (dz * k));
+        const int xs = min(max(x + i, 0), nx - 1);
+        const int ys = min(max(y + j, 0), ny - 1);
+        const int zs = min(max(z + k, 0), nz - 1);
+        const R fs = f[((int64_t)xs * ny + ys) * nz + zs];
+        const R d = fr - fs;
+        const R ad = fabs(d);
+        const R num = d * (gamma * ad + R(3) * fs + fr + R(2) * delta);
+        const R den = fr + fs + gamma * ad + delta;
+        acc = acc + (num / (den * den)) * R(w);
+      }
+  out[idx] = beta_scale * acc;
+}
+
+// PreconditionedGradientAscentAlgorithm.__call__ (preconditioned_gradient_ascent.py:119-122)
+// with BSREM's preconditioner (:461-474).
+template <typename R>
+__global__ __launch_bounds__(256) void bsrem_update_kernel(R *__restrict__ f, const R *__restrict__ b,
+                                                           const R *__restrict__ nrm,
+                                                           const R *__restrict__ nsum,
+                                                           const R *__restrict__ prior,
+                                                           int64_t count, R relax, R weight,
+                                                           R delta, R floor_value) {
+#pragma clang fp contract(off)
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= count) return;
+  const R fo = f[idx];
+  const R d = relax / (weight * nsum[idx] + delta);
+  const R pre = fo * d;
+  const R grad = b[idx] - nrm[idx];
+  const R pg = prior ? prior[idx] : R(0);
+  R v = fo + pre * (grad - pg);
+  if (v <= floor_value) v = floor_value;
+  f[idx] = v;
+}
+
+}  // namespace
+
+template <typename R>
+hipError_t launch_ratio(const R *g, const R *hf, const R *s, R *pred, R *ratio, int64_t count,
+                        R delta, hipStream_t st) {
+  ratio_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(g, hf, s, pred, ratio, count,
+                                                                   delta);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_rdp(const R *f, R *out, int nx, int ny, int nz, double dx, double dy, double dz,
+                      R beta_scale, R gamma, R delta, hipStream_t st) {
+  const int64_t count = (int64_t)nx * ny * nz;
+  rdp_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(f, out, nx, ny, nz, dx, dy, dz,
+                                                                 beta_scale, gamma, delta);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_bsrem_update(R *f, const R *b, const R *nrm, const R *nsum, const R *prior,
+                               int64_t count, R relax, R weight, R delta, R floor_value,
+                               hipStream_t st) {
+  bsrem_update_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(
+      f, b, nrm, nsum, prior, count, relax, weight, delta, floor_value);
+  return hipGetLastError();
+}
+
+#define SPECT_INST(R)                                                                          \
+  template hipError_t launch_ratio<R>(const R *, const R *, const R *, R *, R *, int64_t, R,   \
+                                      hipStream_t);                                            \
+  template hipError_t launch_rdp<R>(const R *, R *, int, int, int, double, double, double, R,  \
+                                    R, R, hipStream_t);                                        \
+  template hipError_t launch_bsrem_update<R>(R *, const R *, const R *, const R *, const R *,  \
+                                             int64_t, R, R, R, R, hipStream_t);
+SPECT_INST(float)
+SPECT_INST(double)
+#undef SPECT_INST
+
+}  // namespace spect

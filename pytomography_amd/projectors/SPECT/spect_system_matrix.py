@@ -41,9 +41,9 @@ class _Plan:
         with torch.cuda.device(device):
             _lib.check(self.lib.spect_plan_create(ctypes.byref(g), ctypes.byref(handle)))
         self.handle = handle
-        padded, pad = ctypes.c_int32(), ctypes.c_int32()
-        _lib.check(self.lib.spect_plan_dims(handle, ctypes.byref(padded), ctypes.byref(pad)))
-        self.padded, self.pad = padded.value, pad.value
+        dims = [ctypes.c_int32() for _ in range(4)]
+        _lib.check(self.lib.spect_plan_dims(handle, *[ctypes.byref(d) for d in dims]))
+        self.padded, self.pad, self.psf_half, self.chunks = (d.value for d in dims)
 
     def __del__(self):
         try:
@@ -73,7 +73,7 @@ class SPECTSystemMatrix(SystemMatrix):
 
     def __init__(self, obj2obj_transforms, proj2proj_transforms, object_meta, proj_meta,
                  object_initial_based_on_camera_path: bool = False,
-                 sensitivity_dtype: torch.dtype = torch.float64) -> None:
+                 sensitivity_dtype: torch.dtype = torch.float64, chunk_planes: int = 0) -> None:
         super().__init__(object_meta, proj_meta, obj2obj_transforms, proj2proj_transforms)
         if object_initial_based_on_camera_path:
             raise NotImplementedError("object_initial_based_on_camera_path is not supported")
@@ -87,10 +87,12 @@ class SPECTSystemMatrix(SystemMatrix):
             raise ValueError(f"projection shape {tuple(proj_meta.shape[1:])} must be (Ly, Lz) = "
                              f"{(shape[1], shape[2])}")
         self._shape = shape
+        self.chunk_planes = int(chunk_planes)
         self._plans: dict = {}
         self._ws: dict = {}
         self._idx32: dict = {}
         self._mu_cache: dict = {}
+        self._att_tables: dict = {}
         self.subset_indices_array = None
 
     # ------------------------------------------------------------------ set-up
@@ -124,7 +126,7 @@ class SPECTSystemMatrix(SystemMatrix):
                     dy=float(self.object_meta.dy), dz=float(self.object_meta.dz),
                     angles_deg=_as_ptr(a32, ctypes.c_float),
                     angles_deg64=_as_ptr(a64, ctypes.c_double),
-                    psf_kr=0, psf_kz=0, n_radii=0)
+                    psf_kr=0, psf_kz=0, n_radii=0, chunk_planes=self.chunk_planes)
         if self._psf is not None:
             radii = np.asarray(self.proj_meta.radii, dtype=np.float64).reshape(-1)
             if radii.size == 1:
@@ -193,9 +195,26 @@ class SPECTSystemMatrix(SystemMatrix):
         mu = self._att.attenuation_map
         cached = self._mu_cache.get(key)
         if cached is None or cached[0] is not mu or cached[2] != mu._version:
-            self._mu_cache[key] = (mu, _lib.require_device_tensor(mu.to(device), "attenuation_map", dt),
+            self._mu_cache[key] = (mu, _lib.require_device_tensor(mu.detach().to(device), "attenuation_map", dt),
                                    mu._version)
+            self._att_tables.pop(key, None)
         return self._mu_cache[key][1]
+
+    def _att_table(self, device, dt):
+        """FP64 attenuation suffix-sum table of the current mu (built once per map)."""
+        mu = self._mu(device, dt)
+        if mu is None:
+            return None
+        key = (device.type, device.index, dt)
+        tab = self._att_tables.get(key)
+        if tab is None:
+            plan = self._plan(device)
+            nbytes = int(plan.lib.spect_att_table_bytes(plan.handle))
+            tab = torch.empty(nbytes // 8, dtype=torch.float64, device=device)
+            _lib.check(plan.lib.spect_att_table_build(plan.handle, _lib.dtype_code(dt), mu.data_ptr(),
+                                                      tab.data_ptr(), _lib.stream_ptr(device)))
+            self._att_tables[key] = tab
+        return tab
 
     @staticmethod
     def _compute_dtype(t: torch.Tensor) -> torch.dtype:
@@ -239,11 +258,13 @@ class SPECTSystemMatrix(SystemMatrix):
         idx = self._angle_index(subset_idx, device)
         n = idx.numel()
         mu = self._mu(device, dt)
+        tab = self._att_table(device, dt)
         out = torch.empty((n, self._shape[1], self._shape[2]), dtype=dt, device=device)
         ws, nbytes = self._workspace(device, dt, n)
         _lib.check(plan.lib.spect_forward(plan.handle, _lib.dtype_code(dt), f.data_ptr(),
-                                          _lib.ptr(mu), idx.data_ptr(), n, out.data_ptr(),
-                                          ws.data_ptr(), nbytes, _lib.stream_ptr(device)))
+                                          _lib.ptr(mu), _lib.ptr(tab), idx.data_ptr(), n,
+                                          out.data_ptr(), ws.data_ptr(), nbytes,
+                                          _lib.stream_ptr(device)))
         return out
 
     def _backward(self, proj, subset_idx, mode, out_mode=_lib.OUT_STORE, out=None, f=None,
@@ -257,12 +278,13 @@ class SPECTSystemMatrix(SystemMatrix):
         if tuple(p.shape) != (n, self._shape[1], self._shape[2]):
             raise ValueError(f"projection shape {tuple(p.shape)} != {(n, self._shape[1], self._shape[2])}")
         mu = self._mu(device, dt)
+        tab = self._att_table(device, dt)
         if out is None:
             out = torch.empty(self._shape, dtype=dt, device=device)
         ws, nbytes = self._workspace(device, dt, n)
         _lib.check(plan.lib.spect_backward(
             plan.handle, _lib.dtype_code(dt), mode, out_mode, p.data_ptr(), _lib.ptr(mu),
-            idx.data_ptr(), n, out.data_ptr(), _lib.ptr(f), _lib.ptr(norm),
+            _lib.ptr(tab), idx.data_ptr(), n, out.data_ptr(), _lib.ptr(f), _lib.ptr(norm),
             float(pytomography_amd.delta), float(floor_value), ws.data_ptr(), nbytes,
             _lib.stream_ptr(device)))
         return out
@@ -278,23 +300,31 @@ class SPECTSystemMatrix(SystemMatrix):
 
     def osem_step(self, object: torch.Tensor, projections: torch.Tensor, additive: torch.Tensor | None,
                   norm: torch.Tensor, subset_idx, pred_out: torch.Tensor | None = None,
-                  floor_value: float = 0.0) -> torch.Tensor:
-        """One OSEM sub-iteration in place on ``object``:
-        pred = H_m f + s; r = g / (pred + delta); f += f/(n+delta) * (H_m^T r - n); clamp."""
+                  floor_value: float = 0.0, keep_prediction: bool = True) -> torch.Tensor | None:
+        """One OSEM sub-iteration in place on ``object`` (a contiguous device
+        tensor): pred = H_m f + s; r = g / (pred + delta);
+        f += f/(n+delta) * (B_m r - n); values <= floor_value become floor_value.
+        Returns pred (``projections_predicted``) unless keep_prediction is False."""
         device = object.device
         dt = object.dtype
-        hf = self.forward(object, subset_idx)
-        ratio = torch.empty_like(hf)
-        if pred_out is None:
-            pred_out = torch.empty_like(hf)
+        if object.device.type != "cuda" or not object.is_contiguous() or object.data_ptr() % 16:
+            raise ValueError("osem_step updates a contiguous, aligned device tensor in place")
+        plan = self._plan(device)
+        idx = self._angle_index(subset_idx, device)
+        n = idx.numel()
         g = _lib.require_device_tensor(projections, "projections", dt)
         s = None if additive is None else _lib.require_device_tensor(additive, "additive_term", dt)
-        _lib.check(_lib.lib().spect_poisson_ratio(
-            _lib.dtype_code(dt), g.data_ptr(), hf.data_ptr(), _lib.ptr(s), pred_out.data_ptr(),
-            ratio.data_ptr(), hf.numel(), float(pytomography_amd.delta), _lib.stream_ptr(device)))
         nrm = _lib.require_device_tensor(norm, "norm", dt)
-        self._backward(ratio, subset_idx, _lib.BP_REFERENCE, _lib.OUT_OSEM, out=object, f=object,
-                       norm=nrm, floor_value=floor_value)
+        mu = self._mu(device, dt)
+        tab = self._att_table(device, dt)
+        if keep_prediction and pred_out is None:
+            pred_out = torch.empty((n, self._shape[1], self._shape[2]), dtype=dt, device=device)
+        ws, nbytes = self._workspace(device, dt, n)
+        _lib.check(plan.lib.spect_osem_subiteration(
+            plan.handle, _lib.dtype_code(dt), object.data_ptr(), _lib.ptr(mu), _lib.ptr(tab),
+            idx.data_ptr(), n, g.data_ptr(), _lib.ptr(s), nrm.data_ptr(),
+            _lib.ptr(pred_out) if keep_prediction else None, float(pytomography_amd.delta),
+            float(floor_value), ws.data_ptr(), nbytes, _lib.stream_ptr(device)))
         return pred_out
 
     def _get_object_initial(self, device=None):
