@@ -39,13 +39,14 @@ __global__ __launch_bounds__(512) void bp_field_kernel(Geo<R> g, Tables<R> tb,
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Src<R> src[2][kTJ];
   const int al = blockIdx.x, jt = blockIdx.y, c = blockIdx.z;
-  const int ang = angle_idx[al];
+  const int ang = ld_uniform(angle_idx + al);
   if ((unsigned)ang >= (unsigned)g.A) return;
   const int tid = threadIdx.x;
   const int Nz = g.Nz;
   const int runs = (Nz + RZ - 1) / RZ;
-  const int ldu = runs * RZ + 1;
-  R *U = reinterpret_cast<R *>(smem);  // [kTJ][ldu] after the y blur
+  const int ldu = runs * RZ + 2 * H + 1;  // H zeros | data | H zeros | pad (odd stride)
+  R *U = reinterpret_cast<R *>(smem);     // [kTJ][ldu] after the y blur, data at +H
+  for (int q = tid; q < kTJ * ldu; q += blockDim.x) U[q] = R(0);
   const int j0 = g.je0 + jt * kTJ;
   const int tj = min(kTJ, g.je1 - j0);
   const int n_jext = g.je1 - g.je0;
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(512) void bp_field_kernel(Geo<R> g, Tables<R> tb,
   const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
   const bool bact = run < runs;
   const bool mine = bact && jl < tj;
-  const auto cs = tb.trig[ang];
+  const auto cs = ld_uniform(tb.trig + ang);
   const bool has_mu = mu != nullptr;
 
   // projection window for the y blur: tile rows [half*16 - H, half*16 + 16 + H)
@@ -80,14 +81,14 @@ __global__ __launch_bounds__(512) void bp_field_kernel(Geo<R> g, Tables<R> tb,
       S[m] = att[(((size_t)rowa * g.C + c) * n_jext + (j0 - g.je0 + jl)) * Nz + z0 + m];
     }
   }
-  const int rix = tb.radius_index[ang];
+  const int rix = ld_uniform(tb.radius_index + ang);
   const int i_lo = c * g.L, i_hi = min(g.P, i_lo + g.L);
   int buf = 0;
   for (int i = i_hi - 1; i >= i_lo; --i) {
     int inside = 0;
     if (tid < tj) {
       R ix, iy;
-      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], tb.lin[i], g.half, ix, iy);
+      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], ld_uniform(tb.lin + i), g.half, ix, iy);
       src[buf][tid] = make_src<R>(ix, iy, g.pad);
       inside = in_footprint<R>(ix, iy, g);
     }
@@ -97,31 +98,33 @@ __global__ __launch_bounds__(512) void bp_field_kernel(Geo<R> g, Tables<R> tb,
       // ---- A: y blur of the projection with this plane's taps -> U
       if (aact) {
         const R *wy = tb.taps_r + ((size_t)rix * g.P + i) * K;
+        R u[HALF_ROWS];
 #pragma unroll
-        for (int r = 0; r < HALF_ROWS; ++r) {
-          R u = R(0);
+        for (int r = 0; r < HALF_ROWS; ++r) u[r] = R(0);
 #pragma unroll
-          for (int t = 0; t < K; ++t) u += wy[t] * pw[r + t];
-          U[(size_t)(half * HALF_ROWS + r) * ldu + za] = u;
+        for (int t = 0; t < K; ++t) {
+          const R w = ld_uniform(wy + t);
+#pragma unroll
+          for (int r = 0; r < HALF_ROWS; ++r) u[r] += w * pw[r + t];
         }
+#pragma unroll
+        for (int r = 0; r < HALF_ROWS; ++r) U[(size_t)(half * HALF_ROWS + r) * ldu + H + za] = u[r];
       }
       __syncthreads();
       // ---- B: z blur of the own run
       if (bact) {
         const R *wz = tb.taps_z + ((size_t)rix * g.P + i) * K;
-        const R *ur = U + (size_t)jl * ldu;
+        const R *ur = U + (size_t)jl * ldu + z0;  // ur[m] = U(z0 - H + m)
         R win[RZ + 2 * H];
 #pragma unroll
-        for (int m = 0; m < RZ + 2 * H; ++m) {
-          const int z = z0 - H + m;
-          win[m] = (z >= 0 && z < Nz) ? ur[z] : R(0);
-        }
+        for (int m = 0; m < RZ + 2 * H; ++m) win[m] = ur[m];
 #pragma unroll
-        for (int m = 0; m < RZ; ++m) {
-          R b = R(0);
+        for (int m = 0; m < RZ; ++m) bz[m] = R(0);
 #pragma unroll
-          for (int t = 0; t < K; ++t) b += wz[t] * win[m + t];
-          bz[m] = b;
+        for (int t = 0; t < K; ++t) {
+          const R w = ld_uniform(wz + t);
+#pragma unroll
+          for (int m = 0; m < RZ; ++m) bz[m] += w * win[m + t];
         }
       }
     } else {
@@ -141,7 +144,7 @@ __global__ __launch_bounds__(512) void bp_field_kernel(Geo<R> g, Tables<R> tb,
       if (has_mu) {
         const Taps4<R> t = taps4<R>(src[buf][jl], g.N, Nz);
         R x[RZ];
-        gather_run<R, RZ>(mu, t, z0, g.N, Nz, x);
+        gather_run<R, RZ>(mu, t, z0, Nz, x);
 #pragma unroll
         for (int m = 0; m < RZ; ++m) {
           const R xm = x[m] * g.dx;
@@ -192,106 +195,146 @@ __device__ __forceinline__ void bp_store(int out_mode, R *__restrict__ out, size
   }
 }
 
-// One wave per object column (x, y); lanes run along z.
-template <typename R>
-__global__ __launch_bounds__(256) void bp_ref_kernel(Geo<R> g, Tables<R> tb,
-                                                     const R *__restrict__ W,
-                                                     const int32_t *__restrict__ angle_idx,
-                                                     int n_call, int out_mode, R *__restrict__ out,
-                                                     const R *__restrict__ f,
-                                                     const R *__restrict__ nrm, R delta,
-                                                     R floor_value) {
-  const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  const int Nz = g.Nz, N = g.N;
-  const int zblocks = (Nz + 63) >> 6;
-  const int col = wave / zblocks;
-  if (col >= N * N) return;
-  const int z = (wave - col * zblocks) * 64 + lane;
-  if (z >= Nz) return;
-  const int x = col / N, y = col - x * N;
-  const int X = x + g.pad, Y = y + g.pad;
-  const int n_jext = g.je1 - g.je0;
-  const R xn = tb.lin[Y], yn = tb.lin[X];
-  R acc = R(0);
-  for (int al = 0; al < n_call; ++al) {
-    const int ang = angle_idx[al];
-    if ((unsigned)ang >= (unsigned)g.A) continue;
-    const auto cs = tb.trig[ang];
-    R ix, iy;
-    rot_coord<R>(cs.x, -cs.y, xn, yn, g.half, ix, iy);  // theta_b = -theta_f
-    const R fx = floor(ix), fy = floor(iy);
-    const int c0 = int(fx), r0 = int(fy);
-    const R tx = ix - fx, ty = iy - fy;
-    const R ex = R(1) - tx, ny = R(1) - ty;
-    const R *Wa = W + (size_t)al * g.P * n_jext * Nz;
-    auto tap = [&](int r, int cc) -> R {
-      if ((unsigned)r >= (unsigned)g.P || cc < g.je0 || cc >= g.je1) return R(0);
-      return Wa[((size_t)r * n_jext + (cc - g.je0)) * Nz + z];
-    };
-    R v = tap(r0, c0) * (ny * ex);
-    v = v + tap(r0, c0 + 1) * (ny * tx);
-    v = v + tap(r0 + 1, c0) * (ty * ex);
-    v = v + tap(r0 + 1, c0 + 1) * (ty * tx);
-    acc = acc + v;
+// Vector of ZV consecutive z values.
+template <typename R, int ZV> struct ZVec { R v[ZV]; };
+
+template <typename R, int ZV>
+__device__ __forceinline__ ZVec<R, ZV> load_zvec(const R *p) {
+  ZVec<R, ZV> r;
+  if constexpr (sizeof(R) == 4 && ZV == 4) {
+    const float4 x = *reinterpret_cast<const float4 *>(p);
+    r.v[0] = x.x; r.v[1] = x.y; r.v[2] = x.z; r.v[3] = x.w;
+  } else if constexpr (sizeof(R) == 8 && ZV == 2) {
+    const double2 x = *reinterpret_cast<const double2 *>(p);
+    r.v[0] = x.x; r.v[1] = x.y;
+  } else {
+#pragma unroll
+    for (int m = 0; m < ZV; ++m) r.v[m] = p[m];
   }
-  bp_store<R>(out_mode, out, ((size_t)x * N + y) * Nz + z, acc, f, nrm, delta, floor_value);
+  return r;
 }
 
-template <typename R>
-__global__ __launch_bounds__(256) void bp_adj_kernel(Geo<R> g, Tables<R> tb,
-                                                     const R *__restrict__ W,
-                                                     const int32_t *__restrict__ angle_idx,
-                                                     int n_call, int out_mode, R *__restrict__ out,
-                                                     const R *__restrict__ f,
-                                                     const R *__restrict__ nrm, R delta,
-                                                     R floor_value) {
-  const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
+template <typename R, int ZV>
+__device__ __forceinline__ void bp_store_vec(int out_mode, R *__restrict__ out, size_t o,
+                                             const R (&b)[ZV], int nvalid,
+                                             const R *__restrict__ f, const R *__restrict__ nrm,
+                                             R delta, R floor_value) {
+#pragma unroll
+  for (int m = 0; m < ZV; ++m)
+    if (m < nvalid) bp_store<R>(out_mode, out, o + m, b[m], f, nrm, delta, floor_value);
+}
+
+// Each thread owns one object column (x, y) and ZV consecutive z; the wave's
+// lanes cover consecutive z (then the next columns), so W reads coalesce.
+// mode 0: reference gather at theta_b = -theta_f (spect_system_matrix.py:196);
+// mode 1: exact transpose of the forward gather.
+template <typename R, int ZV, int MODE>
+__global__ __launch_bounds__(256) void bp_gather_kernel(Geo<R> g, Tables<R> tb,
+                                                        const R *__restrict__ W,
+                                                        const int32_t *__restrict__ angle_idx,
+                                                        int n_call, int out_mode,
+                                                        R *__restrict__ out,
+                                                        const R *__restrict__ f,
+                                                        const R *__restrict__ nrm, R delta,
+                                                        R floor_value) {
   const int Nz = g.Nz, N = g.N;
-  const int zblocks = (Nz + 63) >> 6;
-  const int col = wave / zblocks;
-  if (col >= N * N) return;
-  const int z = (wave - col * zblocks) * 64 + lane;
-  if (z >= Nz) return;
+  const int zq = (Nz + ZV - 1) / ZV;  // z groups per column
+  const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (gid >= (size_t)N * N * zq) return;
+  const int col = (int)(gid / zq);
+  const int z0 = (int)(gid - (size_t)col * zq) * ZV;
+  const int nvalid = min(ZV, Nz - z0);
+  const bool vec = (nvalid == ZV) && (Nz % 4 == 0);
   const int x = col / N, y = col - x * N;
   const int X = x + g.pad, Y = y + g.pad;
   const int n_jext = g.je1 - g.je0;
-  R acc = R(0);
-  for (int al = 0; al < n_call; ++al) {
-    const int ang = angle_idx[al];
-    if ((unsigned)ang >= (unsigned)g.A) continue;
-    const auto cs = tb.trig[ang];
-    // rotated-frame position whose forward sample lands on (row X, col Y)
-    R jc, ic;
-    rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, jc, ic);
-    const int ib = int(floor(ic - R(1.5))), jb = int(floor(jc - R(1.5)));
-    const R *Wa = W + (size_t)al * g.P * n_jext * Nz;
-    R v = R(0);
-    for (int di = 0; di < 4; ++di) {
-      const int i = ib + di;
-      if ((unsigned)i >= (unsigned)g.P) continue;
-      for (int dj = 0; dj < 4; ++dj) {
-        const int j = jb + dj;
-        if (j < g.je0 || j >= g.je1) continue;
-        R ix, iy;
-        rot_coord<R>(cs.x, cs.y, tb.lin[j], tb.lin[i], g.half, ix, iy);
-        const R fx = floor(ix), fy = floor(iy);
-        const int c0 = int(fx), r0 = int(fy);
-        const R tx = ix - fx, ty = iy - fy;
-        R wr, wc;
-        if (r0 == X) wr = R(1) - ty;
-        else if (r0 + 1 == X) wr = ty;
-        else continue;
-        if (c0 == Y) wc = R(1) - tx;
-        else if (c0 + 1 == Y) wc = tx;
-        else continue;
-        v = v + Wa[((size_t)i * n_jext + (j - g.je0)) * Nz + z] * (wr * wc);
-      }
+  const size_t plane = (size_t)n_jext * Nz;
+  R acc[ZV];
+#pragma unroll
+  for (int m = 0; m < ZV; ++m) acc[m] = R(0);
+  auto fetch = [&](const R *p, R (&v)[ZV]) {
+    if (vec) {
+      const ZVec<R, ZV> t = load_zvec<R, ZV>(p);
+#pragma unroll
+      for (int m = 0; m < ZV; ++m) v[m] = t.v[m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < ZV; ++m) v[m] = m < nvalid ? p[m] : R(0);
     }
-    acc = acc + v;
+  };
+  for (int al = 0; al < n_call; ++al) {
+    const int ang = ld_uniform(angle_idx + al);
+    if ((unsigned)ang >= (unsigned)g.A) continue;
+    const auto cs = ld_uniform(tb.trig + ang);
+    const R *Wa = W + (size_t)al * g.P * plane;
+    if constexpr (MODE == 0) {
+      R ix, iy;
+      rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, ix, iy);  // theta_b = -theta_f
+      const R fx = floor(ix), fy = floor(iy);
+      const int c0 = int(fx), r0 = int(fy);
+      const R tx = ix - fx, ty = iy - fy;
+      const R ex = R(1) - tx, ny = R(1) - ty;
+      const int rr[2] = {r0, r0 + 1}, cc[2] = {c0, c0 + 1};
+      const R wr[2] = {ny, ty}, wc[2] = {ex, tx};
+      R v[ZV];
+#pragma unroll
+      for (int m = 0; m < ZV; ++m) v[m] = R(0);
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const bool ok = (unsigned)rr[a] < (unsigned)g.P && cc[b] >= g.je0 && cc[b] < g.je1;
+          const int r = min(max(rr[a], 0), g.P - 1);
+          const int cl = min(max(cc[b], g.je0), g.je1 - 1);
+          const R w = ok ? wr[a] * wc[b] : R(0);
+          R t[ZV];
+          fetch(Wa + (size_t)r * plane + (size_t)(cl - g.je0) * Nz + z0, t);
+          // ((nw + ne) + sw) + se
+#pragma unroll
+          for (int m = 0; m < ZV; ++m) v[m] = (a == 0 && b == 0) ? t[m] * w : v[m] + t[m] * w;
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < ZV; ++m) acc[m] = acc[m] + v[m];
+    } else {
+      // rotated-frame position whose forward sample lands on (row X, col Y)
+      R jc, ic;
+      rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, jc, ic);
+      const int ib = int(floor(ic - R(1.5))), jb = int(floor(jc - R(1.5)));
+      R v[ZV];
+#pragma unroll
+      for (int m = 0; m < ZV; ++m) v[m] = R(0);
+      for (int di = 0; di < 4; ++di) {
+        const int i = ib + di;
+        if ((unsigned)i >= (unsigned)g.P) continue;
+        for (int dj = 0; dj < 4; ++dj) {
+          const int j = jb + dj;
+          if (j < g.je0 || j >= g.je1) continue;
+          R ix, iy;
+          rot_coord<R>(cs.x, cs.y, tb.lin[j], tb.lin[i], g.half, ix, iy);
+          const R fx = floor(ix), fy = floor(iy);
+          const int c0 = int(fx), r0 = int(fy);
+          const R tx = ix - fx, ty = iy - fy;
+          R wr, wc;
+          if (r0 == X) wr = R(1) - ty;
+          else if (r0 + 1 == X) wr = ty;
+          else continue;
+          if (c0 == Y) wc = R(1) - tx;
+          else if (c0 + 1 == Y) wc = tx;
+          else continue;
+          const R w = wr * wc;
+          R t[ZV];
+          fetch(Wa + (size_t)i * plane + (size_t)(j - g.je0) * Nz + z0, t);
+#pragma unroll
+          for (int m = 0; m < ZV; ++m) v[m] = v[m] + t[m] * w;
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < ZV; ++m) acc[m] = acc[m] + v[m];
+    }
   }
-  bp_store<R>(out_mode, out, ((size_t)x * N + y) * Nz + z, acc, f, nrm, delta, floor_value);
+  bp_store_vec<R, ZV>(out_mode, out, ((size_t)x * N + y) * Nz + z0, acc, nvalid, f, nrm, delta,
+                      floor_value);
 }
 
 template <typename R, int H>
@@ -301,7 +344,7 @@ hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, c
   constexpr int RZ = RunLen<R>::value;
   const int bt = tile_block_threads(g.Nz, RZ);
   const int runs = (g.Nz + RZ - 1) / RZ;
-  const size_t smem = (size_t)kTJ * (runs * RZ + 1) * sizeof(R);
+  const size_t smem = (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
   auto kern = bp_field_kernel<R, RZ, H>;
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
@@ -339,15 +382,15 @@ template <typename R>
 hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
                             const int32_t *idx, int n, int mode, int out_mode, R *out, const R *f,
                             const R *nrm, R delta, R floor_value, hipStream_t st) {
-  const int zblocks = (g.Nz + 63) / 64;
-  const size_t waves = (size_t)g.N * g.N * zblocks;
-  const unsigned blocks = (unsigned)((waves * 64 + 255) / 256);
+  constexpr int ZV = sizeof(R) == 4 ? 4 : 2;
+  const size_t threads = (size_t)g.N * g.N * ((g.Nz + ZV - 1) / ZV);
+  const unsigned blocks = (unsigned)((threads + 255) / 256);
   if (mode == SPECT_BP_REFERENCE)
-    bp_ref_kernel<R><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm, delta,
-                                             floor_value);
+    bp_gather_kernel<R, ZV, 0><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm,
+                                                       delta, floor_value);
   else
-    bp_adj_kernel<R><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm, delta,
-                                             floor_value);
+    bp_gather_kernel<R, ZV, 1><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm,
+                                                       delta, floor_value);
   return hipGetLastError();
 }
 

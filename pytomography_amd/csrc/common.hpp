@@ -79,12 +79,27 @@ __device__ __forceinline__ Src<R> make_src(R ix, R iy, int pad) {
   return s;
 }
 
-// Bilinear weights in grid_sample's CPU order: nw=(1-ty)(1-tx), ne=(1-ty)tx,
-// sw=ty(1-tx), se=ty*tx; out-of-object taps weigh zero.
+// Load through the constant address space: a wave-uniform address becomes a
+// scalar (s_load) load.  Only for tables the kernels never write.
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T *p) {
+  return *(const __attribute__((address_space(4))) T *)(p);
+}
+__device__ __forceinline__ float2 ld_uniform(const float2 *p) {
+  const float *q = reinterpret_cast<const float *>(p);
+  return make_float2(ld_uniform(q), ld_uniform(q + 1));
+}
+__device__ __forceinline__ double2 ld_uniform(const double2 *p) {
+  const double *q = reinterpret_cast<const double *>(p);
+  return make_double2(ld_uniform(q), ld_uniform(q + 1));
+}
+
+// Bilinear taps in grid_sample's CPU order: nw=(1-ty)(1-tx), ne=(1-ty)tx,
+// sw=ty(1-tx), se=ty*tx.  Taps outside the object get weight zero and an
+// offset clamped onto a valid voxel, so the loads need no branches.
 template <typename R> struct Taps4 {
-  size_t base;        // element offset of (r0, c0, 0)
-  R wnw, wne, wsw, wse;
-  bool nw, ne, sw, se;
+  uint32_t o[4];  // element offsets of (row, col, 0)
+  R w[4];
 };
 
 template <typename R>
@@ -93,75 +108,58 @@ __device__ __forceinline__ Taps4<R> taps4(const Src<R> &s, int N, int Nz) {
   const R ex = R(1) - s.tx, ny = R(1) - s.ty;
   const bool r0 = (unsigned)s.r0 < (unsigned)N, r1 = (unsigned)(s.r0 + 1) < (unsigned)N;
   const bool c0 = (unsigned)s.c0 < (unsigned)N, c1 = (unsigned)(s.c0 + 1) < (unsigned)N;
-  t.nw = r0 && c0;
-  t.ne = r0 && c1;
-  t.sw = r1 && c0;
-  t.se = r1 && c1;
-  t.wnw = ny * ex;
-  t.wne = ny * s.tx;
-  t.wsw = s.ty * ex;
-  t.wse = s.ty * s.tx;
-  // Offset of (r0, c0); it wraps for negative rows/columns, which is fine
-  // because it is only dereferenced (with unsigned wrap-around arithmetic)
-  // through a tap that lies inside the object.
-  t.base = (size_t)((long long)s.r0 * N + s.c0) * Nz;
+  const int rr0 = min(max(s.r0, 0), N - 1), rr1 = min(max(s.r0 + 1, 0), N - 1);
+  const int cc0 = min(max(s.c0, 0), N - 1), cc1 = min(max(s.c0 + 1, 0), N - 1);
+  t.o[0] = (uint32_t)(rr0 * N + cc0) * (uint32_t)Nz;
+  t.o[1] = (uint32_t)(rr0 * N + cc1) * (uint32_t)Nz;
+  t.o[2] = (uint32_t)(rr1 * N + cc0) * (uint32_t)Nz;
+  t.o[3] = (uint32_t)(rr1 * N + cc1) * (uint32_t)Nz;
+  t.w[0] = (r0 && c0) ? ny * ex : R(0);
+  t.w[1] = (r0 && c1) ? ny * s.tx : R(0);
+  t.w[2] = (r1 && c0) ? s.ty * ex : R(0);
+  t.w[3] = (r1 && c1) ? s.ty * s.tx : R(0);
   return t;
 }
 
 template <typename R>
-__device__ __forceinline__ R interp(const R *__restrict__ vol, const Taps4<R> &t, int z, int N,
-                                    int Nz) {
-  const R vnw = t.nw ? vol[t.base + z] : R(0);
-  const R vne = t.ne ? vol[t.base + Nz + z] : R(0);
-  const R vsw = t.sw ? vol[t.base + (size_t)N * Nz + z] : R(0);
-  const R vse = t.se ? vol[t.base + (size_t)N * Nz + Nz + z] : R(0);
-  R acc = vnw * t.wnw;
-  acc = acc + vne * t.wne;
-  acc = acc + vsw * t.wsw;
-  acc = acc + vse * t.wse;
+__device__ __forceinline__ R interp(const R *__restrict__ vol, const Taps4<R> &t, int z) {
+  R acc = vol[t.o[0] + z] * t.w[0];
+  acc = acc + vol[t.o[1] + z] * t.w[1];
+  acc = acc + vol[t.o[2] + z] * t.w[2];
+  acc = acc + vol[t.o[3] + z] * t.w[3];
   return acc;
 }
 
-// Gather RZ consecutive z of one rotated-frame pixel (bilinear), vectorised
-// when the run is aligned and complete.
+// Gather RZ consecutive z of one rotated-frame pixel (bilinear); 16-byte
+// vector loads when the run is complete and Nz is a multiple of 4.
 template <typename R, int RZ>
 __device__ __forceinline__ void gather_run(const R *__restrict__ vol, const Taps4<R> &t, int z0,
-                                           int N, int Nz, R (&out)[RZ]) {
+                                           int Nz, R (&out)[RZ]) {
   if (z0 + RZ <= Nz && (Nz % 4) == 0) {
 #pragma unroll
-    for (int m = 0; m < RZ; ++m) out[m] = R(0);
-    const size_t offs[4] = {t.base, t.base + Nz, t.base + (size_t)N * Nz, t.base + (size_t)N * Nz + Nz};
-    const bool ok[4] = {t.nw, t.ne, t.sw, t.se};
-    const R w[4] = {t.wnw, t.wne, t.wsw, t.wse};
-#pragma unroll
     for (int q = 0; q < 4; ++q) {
+      const R *p = vol + t.o[q] + z0;
       R v[RZ];
-      if (ok[q]) {
-        const R *p = vol + offs[q] + z0;
-        if constexpr (sizeof(R) == 4) {
+      if constexpr (sizeof(R) == 4) {
 #pragma unroll
-          for (int m = 0; m < RZ; m += 4) {
-            const float4 x = *reinterpret_cast<const float4 *>(p + m);
-            v[m] = x.x; v[m + 1] = x.y; v[m + 2] = x.z; v[m + 3] = x.w;
-          }
-        } else {
-#pragma unroll
-          for (int m = 0; m < RZ; m += 2) {
-            const double2 x = *reinterpret_cast<const double2 *>(p + m);
-            v[m] = x.x; v[m + 1] = x.y;
-          }
+        for (int m = 0; m < RZ; m += 4) {
+          const float4 x = *reinterpret_cast<const float4 *>(p + m);
+          v[m] = x.x; v[m + 1] = x.y; v[m + 2] = x.z; v[m + 3] = x.w;
         }
       } else {
 #pragma unroll
-        for (int m = 0; m < RZ; ++m) v[m] = R(0);
+        for (int m = 0; m < RZ; m += 2) {
+          const double2 x = *reinterpret_cast<const double2 *>(p + m);
+          v[m] = x.x; v[m + 1] = x.y;
+        }
       }
       // ((nw + ne) + sw) + se, as interp()
 #pragma unroll
-      for (int m = 0; m < RZ; ++m) out[m] = q == 0 ? v[m] * w[0] : out[m] + v[m] * w[q];
+      for (int m = 0; m < RZ; ++m) out[m] = q == 0 ? v[m] * t.w[0] : out[m] + v[m] * t.w[q];
     }
   } else {
 #pragma unroll
-    for (int m = 0; m < RZ; ++m) out[m] = (z0 + m < Nz) ? interp<R>(vol, t, z0 + m, N, Nz) : R(0);
+    for (int m = 0; m < RZ; ++m) out[m] = (z0 + m < Nz) ? interp<R>(vol, t, z0 + m) : R(0);
   }
 }
 

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(512) void att_table_kernel(Geo<R> g, Tables<R> tb,
                                                         double *__restrict__ table) {
   __shared__ Src<R> src[kTJ];
   const int al = blockIdx.x, jt = blockIdx.y;
-  const int ang = angle_idx[al];
+  const int ang = ld_uniform(angle_idx + al);
   if ((unsigned)ang >= (unsigned)g.A) return;
   const int tid = threadIdx.x;
   const int j0 = g.je0 + jt * kTJ;
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(512) void att_table_kernel(Geo<R> g, Tables<R> tb,
   const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
   const bool mine = run < runs && jl < tj;
   const int n_jext = g.je1 - g.je0;
-  const auto cs = tb.trig[ang];
+  const auto cs = ld_uniform(tb.trig + ang);
   double S[RZ];
 #pragma unroll
   for (int m = 0; m < RZ; ++m) S[m] = 0.0;
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(512) void att_table_kernel(Geo<R> g, Tables<R> tb,
     int inside = 0;
     if (tid < tj) {
       R ix, iy;
-      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], tb.lin[i], g.half, ix, iy);
+      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], ld_uniform(tb.lin + i), g.half, ix, iy);
       src[tid] = make_src<R>(ix, iy, g.pad);
       inside = in_footprint<R>(ix, iy, g);
     }
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(512) void att_table_kernel(Geo<R> g, Tables<R> tb,
       if (mine) {
         const Taps4<R> t = taps4<R>(src[jl], g.N, g.Nz);
         R x[RZ];
-        gather_run<R, RZ>(mu, t, z0, g.N, g.Nz, x);
+        gather_run<R, RZ>(mu, t, z0, g.Nz, x);
 #pragma unroll
         for (int m = 0; m < RZ; ++m) S[m] += (double)(x[m] * g.dx);
       }
@@ -90,6 +90,11 @@ __global__ __launch_bounds__(512) void att_table_kernel(Geo<R> g, Tables<R> tb,
   }
 }
 
+// LDS rows: H zeros | runs*RZ values | H zeros | 1 pad (odd stride).
+template <int RZ, int H> __device__ __forceinline__ int lds_stride(int runs) {
+  return runs * RZ + 2 * H + 1;
+}
+
 template <typename R, int RZ, int H>
 __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const R *__restrict__ f,
                                                   const R *__restrict__ mu,
@@ -102,14 +107,14 @@ __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const 
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Src<R> src[kTJ];
   const int al = blockIdx.x, jt = blockIdx.y, c = blockIdx.z;
-  const int ang = angle_idx[al];
+  const int ang = ld_uniform(angle_idx + al);
   if ((unsigned)ang >= (unsigned)g.A) return;
   const int tid = threadIdx.x;
   const int Nz = g.Nz;
   const int runs = (Nz + RZ - 1) / RZ;
-  const int ldv = runs * RZ + 1;  // odd row stride: conflict-free column reads
-  R *V = reinterpret_cast<R *>(smem);
-  R *Vz = V + (size_t)kTJ * ldv;
+  const int ldv = lds_stride<RZ, H>(runs);
+  R *V = reinterpret_cast<R *>(smem);         // [kTJ][ldv], data at +H
+  R *Vz = V + (size_t)kTJ * ldv;              // [kTJ][ldv], data at +0
   const int j0 = g.je0 + jt * kTJ;
   const int tj = min(kTJ, g.je1 - j0);
   const int n_jext = g.je1 - g.je0;
@@ -121,9 +126,15 @@ __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const 
   const int nzc = (Nz + 63) / 64 * 64;
   const int half = tid / nzc, zc = tid - half * nzc;
   const bool cact = half < 2 && zc < Nz;
-  const auto cs = tb.trig[ang];
+  const auto cs = ld_uniform(tb.trig + ang);
   const bool has_mu = mu != nullptr;
 
+  if constexpr (H > 0) {  // zero the halo pads of V once
+    for (int q = tid; q < kTJ * 2 * H; q += blockDim.x) {
+      const int r = q / (2 * H), m = q - r * 2 * H;
+      V[(size_t)r * ldv + (m < H ? m : runs * RZ + m)] = R(0);
+    }
+  }
   double S[RZ];
 #pragma unroll
   for (int m = 0; m < RZ; ++m) {
@@ -137,13 +148,13 @@ __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const 
 #pragma unroll
   for (int q = 0; q < JH; ++q) acc[q] = R(0);
 
-  const int rix = tb.radius_index[ang];
+  const int rix = ld_uniform(tb.radius_index + ang);
   const int i_lo = c * g.L, i_hi = min(g.P, i_lo + g.L);
   for (int i = i_hi - 1; i >= i_lo; --i) {
     int inside = 0;
     if (tid < tj) {
       R ix, iy;
-      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], tb.lin[i], g.half, ix, iy);
+      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], ld_uniform(tb.lin + i), g.half, ix, iy);
       src[tid] = make_src<R>(ix, iy, g.pad);
       inside = in_footprint<R>(ix, iy, g);
     }
@@ -153,10 +164,10 @@ __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const 
     R own[RZ];
     if (mine) {
       const Taps4<R> t = taps4<R>(src[jl], g.N, Nz);
-      gather_run<R, RZ>(f, t, z0, g.N, Nz, own);
+      gather_run<R, RZ>(f, t, z0, Nz, own);
       if (has_mu) {
         R x[RZ];
-        gather_run<R, RZ>(mu, t, z0, g.N, Nz, x);
+        gather_run<R, RZ>(mu, t, z0, Nz, x);
 #pragma unroll
         for (int m = 0; m < RZ; ++m) {
           const R xm = x[m] * g.dx;
@@ -186,29 +197,32 @@ __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const 
     } else {
       if (ab) {
 #pragma unroll
-        for (int m = 0; m < RZ; ++m) V[(size_t)jl * ldv + z0 + m] = own[m];
+        for (int m = 0; m < RZ; ++m) V[(size_t)jl * ldv + H + z0 + m] = own[m];
       }
       __syncthreads();
       // ---- B: z blur (correlation, zero padded) of the own run
       const R *wz = tb.taps_z + ((size_t)rix * g.P + i) * K;
       if (ab) {
         R win[RZ + 2 * H];
-        const R *vr = V + (size_t)jl * ldv;
+        const R *vr = V + (size_t)jl * ldv + z0;  // win[m] = V(z0 - H + m)
 #pragma unroll
         for (int m = 0; m < H; ++m) {
-          const int zl = z0 - H + m, zr = z0 + RZ + m;
-          win[m] = zl >= 0 ? vr[zl] : R(0);
-          win[H + RZ + m] = zr < Nz ? vr[zr] : R(0);
+          win[m] = vr[m];
+          win[H + RZ + m] = vr[H + RZ + m];
         }
 #pragma unroll
         for (int m = 0; m < RZ; ++m) win[H + m] = own[m];
+        R out[RZ];
 #pragma unroll
-        for (int m = 0; m < RZ; ++m) {
-          R u = R(0);
+        for (int m = 0; m < RZ; ++m) out[m] = R(0);
 #pragma unroll
-          for (int t = 0; t < K; ++t) u += wz[t] * win[m + t];
-          Vz[(size_t)jl * ldv + z0 + m] = u;
+        for (int t = 0; t < K; ++t) {
+          const R w = ld_uniform(wz + t);
+#pragma unroll
+          for (int m = 0; m < RZ; ++m) out[m] += w * win[m + t];
         }
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) Vz[(size_t)jl * ldv + z0 + m] = out[m];
       }
       __syncthreads();
       // ---- C: y blur as a scatter into the tile's output columns
@@ -220,6 +234,9 @@ __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const 
           constexpr int jhi = (jlo + JH < JO) ? jlo + JH : JO;
           constexpr int jv0 = (jlo - 2 * H > 0) ? jlo - 2 * H : 0;
           constexpr int jv1 = jhi < kTJ ? jhi : kTJ;
+          R w[K];
+#pragma unroll
+          for (int t = 0; t < K; ++t) w[t] = ld_uniform(wy + t);
 #pragma unroll
           for (int jv = jv0; jv < jv1; ++jv) {
             const R v = Vz[(size_t)jv * ldv + zc];
@@ -227,7 +244,7 @@ __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const 
             for (int q = 0; q < JH; ++q) {
               const int jo = jlo + q;
               const int t = jv - jo + 2 * H;  // out[jo] += w[t] * in[jo + t - 2H] (tile-local)
-              if (jo < jhi && t >= 0 && t < K) acc[q] += wy[t] * v;
+              if (jo < jhi && t >= 0 && t < K) acc[q] += w[t] * v;
             }
           }
         };
@@ -291,7 +308,7 @@ hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const 
   constexpr int RZ = RunLen<R>::value;
   const int bt = tile_block_threads(g.Nz, RZ);
   const int runs = (g.Nz + RZ - 1) / RZ;
-  const size_t smem = 2 * (size_t)kTJ * (runs * RZ + 1) * sizeof(R);
+  const size_t smem = 2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
   auto kern = fwd_kernel<R, RZ, H>;
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
