@@ -26,8 +26,8 @@
 namespace spect {
 namespace {
 
-template <typename R, int RZ, int H>
-__global__ __launch_bounds__(512) void bp_field_kernel(Geo<R> g, Tables<R> tb,
+template <typename R, int RZ, int QS, int H>
+__global__ __launch_bounds__(1024) void bp_field_kernel(Geo<R> g, Tables<R> tb,
                                                        const R *__restrict__ proj,
                                                        const R *__restrict__ mu,
                                                        const double *__restrict__ att,
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(512) void bp_field_kernel(Geo<R> g, Tables<R> tb,
                                                        const int32_t *__restrict__ angle_idx,
                                                        R *__restrict__ W) {
   constexpr int K = 2 * H + 1;
-  constexpr int HALF_ROWS = kTJ / 2;
+  constexpr int HALF_ROWS = kTJ / QS;  // tile rows per y-phase thread
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Src<R> src[2][kTJ];
   const int al = blockIdx.x, jt = blockIdx.y, c = blockIdx.z;
@@ -51,10 +51,10 @@ __global__ __launch_bounds__(512) void bp_field_kernel(Geo<R> g, Tables<R> tb,
   const int tj = min(kTJ, g.je1 - j0);
   const int n_jext = g.je1 - g.je0;
   const int nout = g.jo1 - g.jo0;
-  // phase A ownership: one z, half of the tile rows (wave-uniform half)
+  // phase A ownership: one z, 1/QS of the tile rows (wave-uniform split)
   const int nzc = (Nz + 63) / 64 * 64;
   const int half = tid / nzc, za = tid - half * nzc;
-  const bool aact = half < 2 && za < Nz;
+  const bool aact = half < QS && za < Nz;
   // phase B ownership: column jl, z run [z0, z0 + RZ)
   const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
   const bool bact = run < runs;
@@ -341,11 +341,11 @@ template <typename R, int H>
 hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
                           const double *att, bool att_global, const int32_t *idx, int n, R *W,
                           hipStream_t st) {
-  constexpr int RZ = RunLen<R>::value;
-  const int bt = tile_block_threads(g.Nz, RZ);
+  constexpr int RZ = BpTile<R>::RZ, QS = BpTile<R>::QS;
+  const int bt = tile_block_threads(g.Nz, RZ, QS);
   const int runs = (g.Nz + RZ - 1) / RZ;
   const size_t smem = (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
-  auto kern = bp_field_kernel<R, RZ, H>;
+  auto kern = bp_field_kernel<R, RZ, QS, H>;
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);

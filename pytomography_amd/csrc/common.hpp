@@ -24,10 +24,18 @@ template <typename R> struct V2;
 template <> struct V2<float> { using T = float2; };
 template <> struct V2<double> { using T = double2; };
 
-// z values per thread in the gather / z-blur phases (registers hold them).
-template <typename R> struct RunLen;
-template <> struct RunLen<float> { static constexpr int value = 16; };
-template <> struct RunLen<double> { static constexpr int value = 8; };
+// Thread mapping of the tile kernels: in the gather / z-blur phases a thread
+// owns one rotated-frame column and RZ consecutive z values; in the y phase it
+// owns one z value and 1/QS of the tile's rows (back projection) or output
+// columns (forward).
+template <typename R> struct FwdTile {
+  static constexpr int RZ = sizeof(R) == 4 ? 16 : 8;
+  static constexpr int QS = 2;
+};
+template <typename R> struct BpTile {
+  static constexpr int RZ = 8;
+  static constexpr int QS = 4;
+};
 
 // Device tables of a plan in one precision.
 template <typename R> struct Tables {
@@ -163,7 +171,7 @@ __device__ __forceinline__ void gather_run(const R *__restrict__ vol, const Taps
   }
 }
 
-__device__ __forceinline__ float att_exp(float v) { return expf(v); }
+__device__ __forceinline__ float att_exp(float v) { return __expf(v); }
 __device__ __forceinline__ double att_exp(double v) { return exp(v); }
 
 // Reverse-cumsum attenuation factor (utils/misc.py:18, attenuation.py:21):

@@ -25,18 +25,18 @@
 
 namespace spect {
 
-int tile_block_threads(int Nz, int run_len) {
+int tile_block_threads(int Nz, int run_len, int splits) {
   const int runs = (Nz + run_len - 1) / run_len;
   const int nzc = (Nz + 63) / 64 * 64;
   int t = kTJ * runs;
-  if (2 * nzc > t) t = 2 * nzc;
+  if (splits * nzc > t) t = splits * nzc;
   return (t + 63) / 64 * 64;
 }
 
 namespace {
 
 template <typename R, int RZ>
-__global__ __launch_bounds__(512) void att_table_kernel(Geo<R> g, Tables<R> tb,
+__global__ __launch_bounds__(1024) void att_table_kernel(Geo<R> g, Tables<R> tb,
                                                         const R *__restrict__ mu,
                                                         const int32_t *__restrict__ angle_idx,
                                                         double *__restrict__ table) {
@@ -95,7 +95,7 @@ template <int RZ, int H> __device__ __forceinline__ int lds_stride(int runs) {
   return runs * RZ + 2 * H + 1;
 }
 
-template <typename R, int RZ, int H>
+template <typename R, int RZ, int QS, int H>
 __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const R *__restrict__ f,
                                                   const R *__restrict__ mu,
                                                   const double *__restrict__ att, int att_global,
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const 
                                                   R *__restrict__ partial, int n_call) {
   constexpr int K = 2 * H + 1;
   constexpr int JO = kTJ + 2 * H;      // output columns touched by the tile
-  constexpr int JH = (JO + 1) / 2;     // per half
+  constexpr int JH = (JO + QS - 1) / QS;  // per split
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Src<R> src[kTJ];
   const int al = blockIdx.x, jt = blockIdx.y, c = blockIdx.z;
@@ -122,10 +122,10 @@ __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const 
   const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
   const bool ab = run < runs;
   const bool mine = ab && jl < tj;
-  // phase C ownership: one z, half of the output columns (wave-uniform half)
+  // phase C ownership: one z, 1/QS of the output columns (wave-uniform split)
   const int nzc = (Nz + 63) / 64 * 64;
   const int half = tid / nzc, zc = tid - half * nzc;
-  const bool cact = half < 2 && zc < Nz;
+  const bool cact = half < QS && zc < Nz;
   const auto cs = ld_uniform(tb.trig + ang);
   const bool has_mu = mu != nullptr;
 
@@ -234,24 +234,28 @@ __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const 
           constexpr int jhi = (jlo + JH < JO) ? jlo + JH : JO;
           constexpr int jv0 = (jlo - 2 * H > 0) ? jlo - 2 * H : 0;
           constexpr int jv1 = jhi < kTJ ? jhi : kTJ;
-          R w[K];
+          R v[kTJ];
 #pragma unroll
-          for (int t = 0; t < K; ++t) w[t] = ld_uniform(wy + t);
+          for (int jv = jv0; jv < jv1; ++jv) v[jv] = Vz[(size_t)jv * ldv + zc];
+          // out[jo] += w[t] * in[jo + t - 2H] (tile-local); t ascending is jv ascending
 #pragma unroll
-          for (int jv = jv0; jv < jv1; ++jv) {
-            const R v = Vz[(size_t)jv * ldv + zc];
+          for (int t = 0; t < K; ++t) {
+            const R w = ld_uniform(wy + t);
 #pragma unroll
             for (int q = 0; q < JH; ++q) {
               const int jo = jlo + q;
-              const int t = jv - jo + 2 * H;  // out[jo] += w[t] * in[jo + t - 2H] (tile-local)
-              if (jo < jhi && t >= 0 && t < K) acc[q] += w[t] * v;
+              const int jv = jo + t - 2 * H;
+              if (jo < jhi && jv >= jv0 && jv < jv1) acc[q] += w * v[jv];
             }
           }
         };
-        if (half == 0)
-          scatter(std::integral_constant<int, 0>{});
-        else
-          scatter(std::integral_constant<int, 1>{});
+        switch (half) {
+          case 0: scatter(std::integral_constant<int, 0>{}); break;
+          case 1: if constexpr (QS > 1) scatter(std::integral_constant<int, 1>{}); break;
+          case 2: if constexpr (QS > 2) scatter(std::integral_constant<int, 2>{}); break;
+          case 3: if constexpr (QS > 3) scatter(std::integral_constant<int, 3>{}); break;
+          default: break;
+        }
       }
     }
   }
@@ -305,11 +309,11 @@ template <typename R, int H>
 hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *mu,
                         const double *att, bool att_global, const int32_t *idx, int n, R *partial,
                         hipStream_t st) {
-  constexpr int RZ = RunLen<R>::value;
-  const int bt = tile_block_threads(g.Nz, RZ);
+  constexpr int RZ = FwdTile<R>::RZ, QS = FwdTile<R>::QS;
+  const int bt = tile_block_threads(g.Nz, RZ, QS);
   const int runs = (g.Nz + RZ - 1) / RZ;
   const size_t smem = 2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
-  auto kern = fwd_kernel<R, RZ, H>;
+  auto kern = fwd_kernel<R, RZ, QS, H>;
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
@@ -325,7 +329,7 @@ hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const 
 template <typename R>
 hipError_t launch_att_table(const Geo<R> &g, const Tables<R> &tb, const R *mu,
                             const int32_t *angle_idx, int n, double *table, hipStream_t st) {
-  constexpr int RZ = RunLen<R>::value;
+  constexpr int RZ = FwdTile<R>::RZ;
   const int runs = (g.Nz + RZ - 1) / RZ;
   const int bt = (kTJ * runs + 63) / 64 * 64;
   dim3 grid(n, g.ntiles);

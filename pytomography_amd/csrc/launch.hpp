@@ -11,7 +11,7 @@
 namespace spect {
 
 // Threads per block of the tile kernels for a z extent Nz.
-int tile_block_threads(int Nz, int run_len);
+int tile_block_threads(int Nz, int run_len, int splits);
 
 // Suffix sums of mu_rot*dx (FP64) at every chunk boundary, for the angles in
 // angle_idx; row of angle a is (by_global ? angle_idx[a] : a).

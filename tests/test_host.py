@@ -1,0 +1,136 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports every
+declared symbol, argument validation fails loudly, and the host mirror of the
+reference interface (metadata, PSF taps, padding, subsets) matches the oracle."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import spect_oracle as so
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "spect_hip.h")).read()
+    return sorted(set(re.findall(r"^(?:int|size_t|const char \*)\s*(spect_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from pytomography_amd import _lib
+    lib = _lib.lib()
+    names = _declared_functions()
+    assert len(names) >= 14
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(_lib.EXPORTED)
+    assert lib.spect_version() >= 2
+
+
+def test_c_abi_rejects_bad_arguments_without_touching_the_gpu():
+    from pytomography_amd import _lib
+    lib = _lib.lib()
+    angles = (ctypes.c_float * 4)(0, 90, 180, 270)
+    g = _lib.SpectGeom(nx=8, ny=9, nz=4, n_angles=4, dx=0.48, dy=0.48, dz=0.48,
+                       angles_deg=ctypes.cast(angles, ctypes.POINTER(ctypes.c_float)))
+    plan = ctypes.c_void_p()
+    assert lib.spect_plan_create(ctypes.byref(g), ctypes.byref(plan)) == -1
+    assert b"nx must equal ny" in lib.spect_last_error()
+    g.ny = 8
+    g.psf_kr = 4
+    assert lib.spect_plan_create(ctypes.byref(g), ctypes.byref(plan)) == -1
+    assert b"odd" in lib.spect_last_error()
+    assert lib.spect_forward(None, 0, None, None, None, None, 1, None, None, 0, None) == -1
+    assert lib.spect_workspace_bytes(None, 0, 4) == 0
+    assert lib.spect_poisson_ratio(0, None, None, None, None, None, 10, 1e-11, None) == -1
+
+
+def test_no_cpu_fallback():
+    import pytomography_amd
+    from pytomography_amd import _lib
+    from tests.helpers import problem_arrays, product_system_matrix
+    pa = problem_arrays(16, 4, seed=0)
+    sm = product_system_matrix(pa, device="cpu")
+    with pytest.raises(_lib.SpectLibraryError):
+        sm.forward(torch.from_numpy(pa["f"]))
+    pytomography_amd.set_device("cuda" if torch.cuda.is_available() else "cpu")
+
+
+def test_unsupported_transform_orders_raise():
+    from pytomography_amd.metadata.SPECT import SPECTObjectMeta, SPECTProjMeta, SPECTPSFMeta
+    from pytomography_amd.transforms.SPECT import SPECTAttenuationTransform, SPECTPSFTransform
+    from pytomography_amd.projectors.SPECT import SPECTSystemMatrix
+    om = SPECTObjectMeta((0.48,) * 3, (16, 16, 8))
+    pm = SPECTProjMeta((16, 8), (0.48, 0.48), [0.0, 90.0], radii=[25.0, 25.0])
+    att = SPECTAttenuationTransform(torch.zeros((16, 16, 8)))
+    psf = SPECTPSFTransform(SPECTPSFMeta((0.03, 0.1)))
+    with pytest.raises(NotImplementedError):
+        SPECTSystemMatrix([psf, att], [], om, pm)
+    SPECTSystemMatrix([att, psf], [], om, pm)
+    with pytest.raises(ValueError):
+        SPECTSystemMatrix([SPECTAttenuationTransform(torch.zeros((8, 8, 8)))], [], om, pm)
+
+
+def test_metadata_matches_reference_fields():
+    from pytomography_amd.metadata.SPECT import SPECTObjectMeta, SPECTProjMeta
+    om = SPECTObjectMeta((0.48, 0.48, 0.48), (128, 128, 128))
+    assert om.pad_size == 27 and om.padded_shape == (182, 182, 128)
+    pm = SPECTProjMeta((128, 128), (0.48, 0.48), np.arange(0, 360, 3.0), radii=[25.0] * 120)
+    assert pm.shape == (120, 128, 128) and pm.padded_shape == (120, 182, 128)
+    assert pm.angles.dtype == torch.float32 and pm.num_projections == 120
+
+
+def test_psf_taps_match_oracle_bitwise():
+    from pytomography_amd.metadata.SPECT import SPECTObjectMeta, SPECTProjMeta, SPECTPSFMeta
+    from pytomography_amd.transforms.SPECT import SPECTPSFTransform
+    om = SPECTObjectMeta((0.48, 0.48, 0.48), (64, 64, 64))
+    pm = SPECTProjMeta((64, 64), (0.48, 0.48), [0.0], radii=[25.0])
+    t = SPECTPSFTransform(SPECTPSFMeta((0.03, 0.1)))
+    t.configure(om, pm)
+    tr, tz = t.radius_taps(25.0, torch.float32)
+    ref_r, ref_z = so.psf_layers(so.PSFModel((0.03, 0.1)), (64, 64, 64), (0.48,) * 3, 25.0)
+    assert tr.shape == (92, 21) and torch.equal(tr, ref_r) and torch.equal(tz, ref_z)
+
+
+def test_padding_helpers_match_oracle():
+    from pytomography_amd.utils import (compute_pad_size, compute_pad_size_padded, pad_object,
+                                        unpad_object, pad_proj, unpad_proj)
+    for n in (5, 64, 128, 191):
+        assert compute_pad_size(n) == so.compute_pad_size(n)
+        assert compute_pad_size_padded(n + 2 * compute_pad_size(n)) == compute_pad_size(n)
+    x = torch.rand((10, 10, 3))
+    assert torch.equal(pad_object(x), so.pad_object(x))
+    assert torch.equal(pad_object(x, "back_project"), so.pad_object(x, "back_project"))
+    assert torch.equal(unpad_object(pad_object(x)), x)
+    p = torch.rand((4, 10, 3))
+    assert torch.equal(unpad_proj(pad_proj(p)), p)
+
+
+def test_subsets_and_weights():
+    from tests.helpers import problem_arrays, product_system_matrix
+    import pytomography_amd
+    pa = problem_arrays(16, 12, seed=0)
+    sm = product_system_matrix(pa, device="cpu")
+    sm.set_n_subsets(4)
+    ref = so.make_geometry((16, 16, 16), (0.48,) * 3, pa["angles"]).subsets(4)
+    for k in range(4):
+        assert torch.equal(sm.subset_indices_array[k].cpu(), ref[k])
+        assert sm.get_weighting_subset(k) == 3 / 12
+    assert sm.get_weighting_subset(None) == 1
+    proj = torch.arange(12 * 2 * 2).reshape(12, 2, 2)
+    assert torch.equal(sm.get_projection_subset(proj, 1), proj[1::4])
+    pytomography_amd.set_device("cuda" if torch.cuda.is_available() else "cpu")
+
+
+def test_synthetic_inputs_are_seeded():
+    from pytomography_amd import synthetic
+    a = synthetic.ellipsoid_phantom(32, 3)
+    b = synthetic.ellipsoid_phantom(32, 3)
+    assert np.array_equal(a, b) and a.max() > 0.1 and a.min() == 0
+    mu = synthetic.attenuation_map(32, 3)
+    assert set(np.unique(mu)) <= {0.0, np.float32(0.15), np.float32(0.25)}

@@ -1,0 +1,245 @@
+"""Pins of the CPU oracle (no GPU).
+
+The reference's own test suite pins no number on this path (its single test
+needs external DICOM data), so the oracle is pinned by closed forms, by the
+numpy float64 brute force in oracle/brute.py (written from the mathematics,
+not from the torch op sequence) and by invariants chosen so that a dropped
+term, a wrong sign or index, or a transposed operand fails at least one test.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import brute
+from oracle import spect_oracle as so
+
+pytestmark = pytest.mark.filterwarnings("ignore::UserWarning")
+
+
+# ----------------------------------------------------------------- padding
+def test_pad_size_closed_form():
+    for n in (1, 2, 16, 64, 128, 192, 1000):
+        assert so.compute_pad_size(n) == math.ceil((math.sqrt(2) * n - n) / 2)
+    # Appendix-B values of the benchmark geometries
+    assert [so.compute_pad_size(n) for n in (64, 96, 128, 160, 192)] == [14, 20, 27, 34, 40]
+
+
+def test_pad_size_roundtrip():
+    for n in range(2, 4001):
+        p = so.compute_pad_size(n)
+        assert so.compute_pad_size_padded(n + 2 * p) == p
+
+
+def test_back_project_box():
+    box = so.pad_object(torch.ones((6, 6, 3)), mode="back_project")
+    p = so.compute_pad_size(6)
+    assert box.shape == (6 + 2 * p, 6 + 2 * p, 3)
+    assert torch.all(box[:, p:p + 6] == 1) and torch.all(box[:, :p] == 0) and torch.all(box[:, p + 6:] == 0)
+
+
+# ----------------------------------------------------------------- rotation
+def _rand(shape, seed=0, dtype=torch.float64):
+    return torch.rand(shape, generator=torch.Generator().manual_seed(seed), dtype=dtype)
+
+
+def test_rotation_zero_is_identity():
+    x = _rand((9, 9, 4))
+    out = so.rotate_xyz(x, torch.tensor(0.0, dtype=torch.float64))
+    assert torch.allclose(out, x, atol=1e-12)
+
+
+@pytest.mark.parametrize("theta,k", [(90.0, 1), (180.0, 2), (270.0, 3)])
+def test_rotation_quarter_turns_are_permutations(theta, k):
+    x = _rand((9, 9, 3), seed=1)
+    out = so.rotate_xyz(x, torch.tensor(theta, dtype=torch.float64))
+    # rows = x, columns = y; kornia's positive angle turns the image counter-clockwise
+    ref = torch.rot90(x, k=k, dims=(0, 1))
+    assert torch.allclose(out, ref, atol=1e-10)
+
+
+def test_rotation_matches_brute_force_nonsquare():
+    x = _rand((5, 7, 3), seed=2)
+    for theta in (13.0, -37.0, 123.0):
+        out = so.rotate_xyz(x, torch.tensor(theta, dtype=torch.float64)).numpy()
+        ref = brute.rotate_volume(x.numpy(), theta)
+        assert np.abs(out - ref).max() < 1e-12
+
+
+def test_rotation_reproduces_linear_ramps_and_constants():
+    n = 21
+    i = torch.arange(n, dtype=torch.float64)
+    ramp = (0.3 * i[:, None] - 0.7 * i[None, :] + 2.0).unsqueeze(-1).repeat(1, 1, 2)
+    theta = 29.0
+    out = so.rotate_xyz(ramp, torch.tensor(theta, dtype=torch.float64))
+    c, s = math.cos(math.radians(theta)), math.sin(math.radians(theta))
+    cc = (n - 1) / 2
+    r = i[:, None].expand(n, n)
+    q = i[None, :].expand(n, n)
+    col = c * (q - cc) - s * (r - cc) + cc
+    row = s * (q - cc) + c * (r - cc) + cc
+    expect = 0.3 * row - 0.7 * col + 2.0
+    inside = (row >= 0) & (row <= n - 1) & (col >= 0) & (col <= n - 1)
+    assert torch.allclose(out[..., 0][inside], expect[inside], atol=1e-10)
+    const = torch.ones((n, n, 2), dtype=torch.float64)
+    outc = so.rotate_xyz(const, torch.tensor(theta, dtype=torch.float64))[..., 0]
+    disk = (r - cc) ** 2 + (q - cc) ** 2 <= (cc - 1) ** 2
+    assert torch.allclose(outc[disk], torch.ones_like(outc[disk]), atol=1e-12)
+
+
+def test_rotation_float32_matches_float64_closely():
+    x = _rand((30, 30, 4), seed=3)
+    a = so.rotate_xyz(x.float(), torch.tensor(-57.0))
+    b = so.rotate_xyz(x, torch.tensor(-57.0, dtype=torch.float64))
+    assert float((a.double() - b).abs().max()) < 1e-5
+
+
+# -------------------------------------------------------------- attenuation
+def test_cumsum_is_float64_accumulated():
+    x = torch.rand((182, 7, 5), generator=torch.Generator().manual_seed(4)) * 0.07
+    c32 = torch.cumsum(x, 0)
+    c64 = torch.cumsum(x.double(), 0).float()
+    assert torch.equal(c32, c64)
+
+
+def test_uniform_mu_attenuation_closed_form():
+    p, mu, dx = 17, 0.15, 0.48
+    m = torch.full((p, 3, 2), mu, dtype=torch.float64)
+    a = so.prob_of_detection(m, dx)
+    i = torch.arange(p, dtype=torch.float64)
+    expect = torch.exp(-mu * dx * (p - i - 0.5))
+    assert torch.allclose(a[:, 1, 1], expect, rtol=1e-14)
+
+
+def test_attenuation_matches_brute_force():
+    m = _rand((11, 4, 3), seed=5) * 0.3
+    a = so.prob_of_detection(m, 0.48).numpy()
+    assert np.abs(a - brute.attenuation(m.numpy(), 0.48)).max() < 1e-14
+
+
+# ----------------------------------------------------------------------- PSF
+def test_psf_kernel_sizes_appendix_b():
+    psf = so.PSFModel((0.03, 0.1))
+    ks = [so.psf_kernel_size(psf, n, 25.0, 0.48, 0.48) for n in (64, 96, 128, 160, 192)]
+    assert ks == [21, 25, 29, 33, 37]
+
+
+def test_psf_sigma_and_taps_closed_form():
+    psf = so.PSFModel((0.03, 0.1))
+    n, r, dx = 128, 25.0, 0.48
+    sig = so.psf_sigma(psf, n, r, dx)
+    p = n + 2 * so.compute_pad_size(n)
+    i = np.arange(p)
+    d = np.maximum(r + (p / 2 - i - 0.5) * dx, 0)
+    assert np.allclose(sig, 0.03 * d + 0.1, rtol=0, atol=1e-15)
+    taps = so.gaussian_taps(sig / dx, 29, torch.float64).numpy()
+    t = np.arange(-14, 15)
+    s = sig[:, None] / dx
+    ref = np.exp(-t ** 2 / (2 * s ** 2 + 1e-11))
+    ref /= ref.sum(1, keepdims=True)
+    assert np.allclose(taps, ref, rtol=1e-13, atol=0)
+    assert np.allclose(taps, taps[:, ::-1])
+
+
+def test_psf_blur_matches_scipy_and_is_self_adjoint():
+    from scipy.ndimage import correlate1d
+    rng = np.random.default_rng(6)
+    x = rng.random((5, 9, 7))
+    taps_r = torch.from_numpy(rng.random((5, 5)))
+    taps_z = torch.from_numpy(rng.random((5, 3)))
+    taps_r = (taps_r + taps_r.flip(1)) / 2
+    taps_z = (taps_z + taps_z.flip(1)) / 2
+    out = so.psf_apply(torch.from_numpy(x), taps_r, taps_z).numpy()
+    ref = np.stack([correlate1d(correlate1d(x[i], taps_r[i].numpy(), axis=0, mode="constant"),
+                                taps_z[i].numpy(), axis=1, mode="constant") for i in range(5)])
+    assert np.abs(out - ref).max() < 1e-13
+    assert np.abs(out - brute.psf(x, taps_r.numpy(), taps_z.numpy())).max() < 1e-13
+    y = rng.random(x.shape)
+    lhs = (out * y).sum()
+    rhs = (x * so.psf_apply(torch.from_numpy(y), taps_r, taps_z).numpy()).sum()
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_psf_delta_gives_kernel():
+    taps = so.gaussian_taps(np.full(3, 1.7), 9, torch.float64)
+    x = torch.zeros((3, 11, 11), dtype=torch.float64)
+    x[:, 5, 5] = 1
+    out = so.psf_apply(x, taps, taps)
+    assert torch.allclose(out[1, 1:10, 1:10], torch.outer(taps[1], taps[1]), atol=1e-15)
+
+
+# --------------------------------------------------------- system matrix
+def _tiny(att=True, psf=True):
+    rng = np.random.default_rng(7)
+    shape = (8, 8, 4)
+    f = rng.random(shape)
+    mu = rng.random(shape) * 0.2 if att else None
+    angles = [0.0, 33.0, 90.0, 141.0, 222.0, 300.0]
+    g = so.make_geometry(shape, (0.48,) * 3, angles, radii=[6.0] * 6,
+                         mu=None if mu is None else torch.from_numpy(mu),
+                         psf=so.PSFModel((0.03, 0.1)) if psf else None, dtype=torch.float64)
+    taps = None
+    if psf:
+        taps = [tuple(None if t is None else t.numpy() for t in g.layers(a)) for a in range(6)]
+    return g, f, mu, angles, taps
+
+
+@pytest.mark.parametrize("att,psf", [(False, False), (True, False), (False, True), (True, True)])
+def test_forward_backward_match_brute_force(att, psf):
+    g, f, mu, angles, taps = _tiny(att, psf)
+    out = so.forward(g, torch.from_numpy(f)).numpy()
+    ref = brute.forward(f, angles, 0.48, mu, taps)
+    assert np.abs(out - ref).max() < 1e-13 * np.abs(ref).max()
+    y = np.random.default_rng(8).random(out.shape)
+    bo = so.backward(g, torch.from_numpy(y)).numpy()
+    bb = brute.backward_ref(y, angles, 0.48, g.shape, mu, taps)
+    assert np.abs(bo - bb).max() < 1e-13 * np.abs(bb).max()
+
+
+def test_exact_adjoint_is_dense_transpose_and_reference_backward_is_not():
+    g, f, mu, angles, taps = _tiny(True, True)
+    H = brute.dense_matrix(lambda e: brute.forward(e, angles, 0.48, mu, taps), g.shape)
+    out = so.forward(g, torch.from_numpy(f)).numpy().ravel()
+    assert np.abs(out - H @ f.ravel()).max() < 1e-12
+    y = np.random.default_rng(9).random((6, 8, 4))
+    adj = so.exact_adjoint(g, torch.from_numpy(y)).numpy().ravel()
+    assert np.abs(adj - H.T @ y.ravel()).max() < 1e-12
+    ref_bp = so.backward(g, torch.from_numpy(y)).numpy().ravel()
+    assert np.abs(ref_bp - H.T @ y.ravel()).max() > 1e-4  # the reference back projection is not H^T
+
+
+def test_osem_one_subset_is_mlem_on_dense_matrix():
+    g, f, mu, angles, taps = _tiny(False, False)
+    H = brute.dense_matrix(lambda e: brute.forward(e, angles, 0.48, None, None), g.shape)
+    B = brute.dense_matrix(lambda p: brute.backward_ref(p, angles, 0.48, g.shape, None, None),
+                           (6, 8, 4))
+    data = H @ (f.ravel() * 3)
+    x = np.ones(f.size)
+    sens = B @ np.ones(data.size)
+    for _ in range(3):
+        x = x + x / (sens + 1e-11) * (B @ (data / (H @ x + 1e-11)) - sens)
+        x[x <= 0] = 0
+    out = so.osem(g, torch.from_numpy(data.reshape(6, 8, 4)), 3, 1).numpy().ravel()
+    assert np.abs(out - x).max() < 1e-10 * np.abs(x).max()
+
+
+# ----------------------------------------------------------------- prior
+def test_rdp_constant_image_has_zero_gradient():
+    f = torch.full((6, 6, 6), 2.5, dtype=torch.float64)
+    assert torch.all(so.rdp_gradient(f, 0.1, 2.0, (0.48,) * 3, 1.0) == 0)
+
+
+def test_rdp_gradient_is_half_fd_of_value_in_interior():
+    f = _rand((7, 7, 7), seed=10) + 0.5
+    beta, gamma = 0.3, 2.0
+    grad = so.rdp_gradient(f, beta, gamma, (0.48,) * 3, 1.0)
+    h = 1e-6
+    for idx in [(3, 3, 3), (2, 4, 3), (4, 2, 2)]:
+        fp, fm = f.clone(), f.clone()
+        fp[idx] += h
+        fm[idx] -= h
+        fd = (so.rdp_value(fp, beta, gamma, (0.48,) * 3) - so.rdp_value(fm, beta, gamma, (0.48,) * 3)) / (2 * h)
+        assert abs(float(fd) - 2 * float(grad[idx])) < 1e-6 * abs(float(fd)) + 1e-9
