@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--subsets", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run one plain device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -140,6 +142,13 @@ def main():
     from pytomography_amd.algorithms import OSEM
     from pytomography_amd.likelihoods import PoissonLogLikelihood
 
+    if args.copy_mib > 0:
+        src = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=device).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     prob = build_problem(n, na, att, psf, seed=0, device=device)
     sm = prob["sm"]
     lk = PoissonLogLikelihood(sm, prob["data"], additive_term=prob["scatter"])
@@ -184,10 +193,9 @@ def main():
     bwd_ms = float(np.mean([ev[k][1].elapsed_time(ev[k][2]) for k in range(ns)]))
     a_s = na // ns
     fwd_b, bwd_b, upd_b, ratio_b = algorithmic_bytes(n, a_s, att)
-    if fwd_ms >= bwd_ms:
-        kern, bytes_launch, ms = "spect_forward (fwd_kernel)", fwd_b, fwd_ms
-    else:
-        kern, bytes_launch, ms = "spect_backward (bww_kernel + bw_ref_kernel, OSEM update)", bwd_b + upd_b, bwd_ms
+    # Dominant kernel: the forward projection (fwd_kernel + its chunk reduce),
+    # timed above with HIP events on the stream it is launched on.
+    kern, bytes_launch, ms = "fwd_kernel + fwd_reduce_kernel (spect_forward, 15 angles)", fwd_b, fwd_ms
     achieved = bytes_launch / (ms * 1e-3) / 1e9
     iter_bytes = ns * (fwd_b + bwd_b + upd_b + ratio_b)
 
@@ -195,7 +203,9 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get(f"{args.workload}_{n}_{na}_{ns}")
+            entry = json.load(fh).get(f"{args.workload}_{n}_{na}_{ns}")
+        if entry:
+            traffic = entry.get("spect_forward")
 
     cpu = None
     if not args.no_cpu_baseline:
