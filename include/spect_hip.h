@@ -140,6 +140,20 @@ int spect_osem_subiteration(const spect_plan *p, int dtype, void *f, const void 
                             double floor_value, void *workspace, size_t workspace_bytes,
                             void *stream);
 
+/* Vector-Jacobian product of the forward projection proj = H(mu) f for the
+ * differentiable projector (no counterpart in the reference, whose transforms
+ * all run under torch.no_grad: transforms/SPECT/attenuation.py:72,94,
+ * transforms/SPECT/psf.py:199,216):
+ *   grad_f  = H(mu)^T u       (exact transpose of spect_forward)
+ *   grad_mu = (d proj / d mu)^T u   (through the reverse cumsum and exp of
+ *             utils/misc.py:18, attenuation.py:21; NULL to skip).
+ * u is [n][ny][nz]; grad_f, grad_mu are [nx][ny][nz]; the attenuation rows
+ * are rebuilt from mu (mu may change between calls).                    */
+size_t spect_grad_workspace_bytes(const spect_plan *p, int dtype, int n);
+int spect_forward_vjp(const spect_plan *p, int dtype, const void *u, const void *f, const void *mu,
+                      const int32_t *angle_idx, int n, void *grad_f, void *grad_mu,
+                      void *workspace, size_t workspace_bytes, void *stream);
+
 /* Poisson ratio of PoissonLogLikelihood.compute_gradient
  * (likelihoods/poisson_log_likelihood.py:34-36):
  *   pred = hf + s (s may be NULL);  ratio = g / (pred + delta).

@@ -23,6 +23,7 @@ EXPORTED = (
     "spect_last_error", "spect_version", "spect_plan_create", "spect_plan_destroy",
     "spect_plan_dims", "spect_workspace_bytes", "spect_att_table_bytes", "spect_att_table_build",
     "spect_forward", "spect_backward", "spect_osem_subiteration",
+    "spect_grad_workspace_bytes", "spect_forward_vjp",
     "spect_poisson_ratio", "spect_rdp_gradient", "spect_bsrem_update",
 )
 
@@ -77,6 +78,10 @@ def _declare(lib):
     lib.spect_backward.argtypes = [vp, ci, ci, ci, vp, vp, vp, vp, ci, vp, vp, vp, dbl, dbl, vp, sz, vp]
     lib.spect_osem_subiteration.restype = ci
     lib.spect_osem_subiteration.argtypes = [vp, ci, vp, vp, vp, vp, ci, vp, vp, vp, vp, dbl, dbl, vp, sz, vp]
+    lib.spect_grad_workspace_bytes.restype = sz
+    lib.spect_grad_workspace_bytes.argtypes = [vp, ci, ci]
+    lib.spect_forward_vjp.restype = ci
+    lib.spect_forward_vjp.argtypes = [vp, ci, vp, vp, vp, vp, ci, vp, vp, vp, sz, vp]
     lib.spect_poisson_ratio.restype = ci
     lib.spect_poisson_ratio.argtypes = [ci, vp, vp, vp, vp, vp, i64, dbl, vp]
     lib.spect_rdp_gradient.restype = ci

@@ -26,14 +26,22 @@
 namespace spect {
 namespace {
 
-template <typename R, int RZ, int QS, int H>
+// GRAD: also the attenuation-map gradient field of the differentiable
+// projector: with D(i) = W(i) * f_rot(i) along a ray, dL/dM(k) =
+// sum_{i>k} D(i) + D(k)/2 - sum_i D(i)  (M = mu_rot*dx).  The block writes the
+// chunk-local part Wm(k) = sum_{i>k, i in chunk} D(i) + D(k)/2 (FP64) and the
+// chunk total Dsum (FP64); the gather subtracts the prefix of chunk totals.
+template <typename R, int RZ, int QS, int H, bool GRAD>
 __global__ __launch_bounds__(1024) void bp_field_kernel(Geo<R> g, Tables<R> tb,
                                                        const R *__restrict__ proj,
                                                        const R *__restrict__ mu,
                                                        const double *__restrict__ att,
                                                        int att_global,
                                                        const int32_t *__restrict__ angle_idx,
-                                                       R *__restrict__ W) {
+                                                       R *__restrict__ W,
+                                                       const R *__restrict__ f,
+                                                       double *__restrict__ Wm,
+                                                       double *__restrict__ Dsum) {
   constexpr int K = 2 * H + 1;
   constexpr int HALF_ROWS = kTJ / QS;  // tile rows per y-phase thread
   extern __shared__ __align__(16) unsigned char smem[];
@@ -81,6 +89,9 @@ __global__ __launch_bounds__(1024) void bp_field_kernel(Geo<R> g, Tables<R> tb,
       S[m] = att[(((size_t)rowa * g.C + c) * n_jext + (j0 - g.je0 + jl)) * Nz + z0 + m];
     }
   }
+  double SD[GRAD ? RZ : 1];
+#pragma unroll
+  for (int m = 0; m < (GRAD ? RZ : 1); ++m) SD[m] = 0.0;
   const int rix = ld_uniform(tb.radius_index + ang);
   const int i_lo = c * g.L, i_hi = min(g.P, i_lo + g.L);
   int buf = 0;
@@ -141,8 +152,8 @@ __global__ __launch_bounds__(1024) void bp_field_kernel(Geo<R> g, Tables<R> tb,
     }
     // ---- attenuation and store
     if (mine) {
+      Taps4<R> t = taps4<R>(src[buf][jl], g.N, Nz);
       if (has_mu) {
-        const Taps4<R> t = taps4<R>(src[buf][jl], g.N, Nz);
         R x[RZ];
         gather_run<R, RZ>(mu, t, z0, Nz, x);
 #pragma unroll
@@ -150,6 +161,17 @@ __global__ __launch_bounds__(1024) void bp_field_kernel(Geo<R> g, Tables<R> tb,
           const R xm = x[m] * g.dx;
           S[m] += (double)xm;
           bz[m] = bz[m] * att_factor<R>(S[m], xm);
+        }
+      }
+      if constexpr (GRAD) {
+        R fv[RZ];
+        gather_run<R, RZ>(f, t, z0, Nz, fv);
+        double *dm = Wm + (((size_t)al * g.P + i) * n_jext + (j0 - g.je0 + jl)) * Nz + z0;
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) {
+          const double d = (double)bz[m] * (double)fv[m];
+          if (z0 + m < Nz) dm[m] = SD[m] + 0.5 * d;
+          SD[m] += d;
         }
       }
       R *dst = W + (((size_t)al * g.P + i) * n_jext + (j0 - g.je0 + jl)) * Nz + z0;
@@ -170,6 +192,28 @@ __global__ __launch_bounds__(1024) void bp_field_kernel(Geo<R> g, Tables<R> tb,
       }
     }
     buf ^= 1;
+  }
+  if constexpr (GRAD) {
+    if (mine) {
+#pragma unroll
+      for (int m = 0; m < RZ; ++m)
+        if (z0 + m < Nz)
+          Dsum[(((size_t)al * g.C + c) * n_jext + (j0 - g.je0 + jl)) * Nz + z0 + m] = SD[m];
+    }
+  }
+}
+
+// In place: Dsum[a][c] <- sum_{c' <= c} Dsum[a][c'] (prefix over chunks, FP64).
+__global__ __launch_bounds__(256) void chunk_prefix_kernel(double *__restrict__ Dsum, int n, int C,
+                                                           size_t plane) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)n * plane) return;
+  const size_t a = idx / plane, r = idx - a * plane;
+  double s = 0.0;
+  for (int c = 0; c < C; ++c) {
+    double &v = Dsum[(a * C + c) * plane + r];
+    s += v;
+    v = s;
   }
 }
 
@@ -228,6 +272,8 @@ __device__ __forceinline__ void bp_store_vec(int out_mode, R *__restrict__ out, 
 // lanes cover consecutive z (then the next columns), so W reads coalesce.
 // mode 0: reference gather at theta_b = -theta_f (spect_system_matrix.py:196);
 // mode 1: exact transpose of the forward gather.
+// mode 2: exact transpose as mode 1 for W -> out, and for the attenuation
+// gradient field dx * (Wm - prefix of chunk totals) -> out_mu (FP64 sums).
 template <typename R, int ZV, int MODE>
 __global__ __launch_bounds__(256) void bp_gather_kernel(Geo<R> g, Tables<R> tb,
                                                         const R *__restrict__ W,
@@ -236,7 +282,10 @@ __global__ __launch_bounds__(256) void bp_gather_kernel(Geo<R> g, Tables<R> tb,
                                                         R *__restrict__ out,
                                                         const R *__restrict__ f,
                                                         const R *__restrict__ nrm, R delta,
-                                                        R floor_value) {
+                                                        R floor_value,
+                                                        const double *__restrict__ Wm,
+                                                        const double *__restrict__ Pref,
+                                                        R *__restrict__ out_mu) {
   const int Nz = g.Nz, N = g.N;
   const int zq = (Nz + ZV - 1) / ZV;  // z groups per column
   const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -250,8 +299,11 @@ __global__ __launch_bounds__(256) void bp_gather_kernel(Geo<R> g, Tables<R> tb,
   const int n_jext = g.je1 - g.je0;
   const size_t plane = (size_t)n_jext * Nz;
   R acc[ZV];
+  double accm[MODE == 2 ? ZV : 1];
 #pragma unroll
   for (int m = 0; m < ZV; ++m) acc[m] = R(0);
+#pragma unroll
+  for (int m = 0; m < (MODE == 2 ? ZV : 1); ++m) accm[m] = 0.0;
   auto fetch = [&](const R *p, R (&v)[ZV]) {
     if (vec) {
       const ZVec<R, ZV> t = load_zvec<R, ZV>(p);
@@ -327,6 +379,14 @@ __global__ __launch_bounds__(256) void bp_gather_kernel(Geo<R> g, Tables<R> tb,
           fetch(Wa + (size_t)i * plane + (size_t)(j - g.je0) * Nz + z0, t);
 #pragma unroll
           for (int m = 0; m < ZV; ++m) v[m] = v[m] + t[m] * w;
+          if constexpr (MODE == 2) {
+            const size_t o = (size_t)i * plane + (size_t)(j - g.je0) * Nz + z0;
+            const double *wm = Wm + (size_t)al * g.P * plane + o;
+            const double *pr = Pref + ((size_t)al * g.C + i / g.L) * plane + (size_t)(j - g.je0) * Nz + z0;
+#pragma unroll
+            for (int m = 0; m < ZV; ++m)
+              if (m < nvalid) accm[m] += (double)w * (wm[m] - pr[m]);
+          }
         }
       }
 #pragma unroll
@@ -335,24 +395,42 @@ __global__ __launch_bounds__(256) void bp_gather_kernel(Geo<R> g, Tables<R> tb,
   }
   bp_store_vec<R, ZV>(out_mode, out, ((size_t)x * N + y) * Nz + z0, acc, nvalid, f, nrm, delta,
                       floor_value);
+  if constexpr (MODE == 2) {
+    R gm[ZV];
+#pragma unroll
+    for (int m = 0; m < ZV; ++m) gm[m] = R((double)g.dx * accm[m]);
+    bp_store_vec<R, ZV>(out_mode, out_mu, ((size_t)x * N + y) * Nz + z0, gm, nvalid, f, nrm, delta,
+                        floor_value);
+  }
 }
 
 template <typename R, int H>
 hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
                           const double *att, bool att_global, const int32_t *idx, int n, R *W,
-                          hipStream_t st) {
+                          const R *f, double *Wm, double *Dsum, hipStream_t st) {
   constexpr int RZ = BpTile<R>::RZ, QS = BpTile<R>::QS;
   const int bt = tile_block_threads(g.Nz, RZ, QS);
   const int runs = (g.Nz + RZ - 1) / RZ;
   const size_t smem = (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
-  auto kern = bp_field_kernel<R, RZ, QS, H>;
-  if (smem > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void *)kern,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (e != hipSuccess) return e;
-  }
   dim3 grid(n, g.ntiles, g.C);
-  kern<<<grid, bt, smem, st>>>(g, tb, proj, mu, att, att_global ? 1 : 0, idx, W);
+  if (Wm) {
+    auto kern = bp_field_kernel<R, RZ, QS, H, true>;
+    if (smem > 64 * 1024) {
+      hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return e;
+    }
+    kern<<<grid, bt, smem, st>>>(g, tb, proj, mu, att, att_global ? 1 : 0, idx, W, f, Wm, Dsum);
+  } else {
+    auto kern = bp_field_kernel<R, RZ, QS, H, false>;
+    if (smem > 64 * 1024) {
+      hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return e;
+    }
+    kern<<<grid, bt, smem, st>>>(g, tb, proj, mu, att, att_global ? 1 : 0, idx, W, nullptr,
+                                 nullptr, nullptr);
+  }
   return hipGetLastError();
 }
 
@@ -361,46 +439,55 @@ hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, c
 template <typename R>
 hipError_t launch_bp_field(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
                            const double *att, bool att_global, const int32_t *idx, int n, R *W,
-                           hipStream_t st) {
+                           const R *f, double *Wm, double *Dsum, hipStream_t st) {
+#define SPECT_FIELD_CASE(HH) \
+  case HH: return launch_field_h<R, HH>(g, tb, proj, mu, att, att_global, idx, n, W, f, Wm, Dsum, st);
   switch (g.H) {
-    case 0: return launch_field_h<R, 0>(g, tb, proj, mu, att, att_global, idx, n, W, st);
-    case 2: return launch_field_h<R, 2>(g, tb, proj, mu, att, att_global, idx, n, W, st);
-    case 4: return launch_field_h<R, 4>(g, tb, proj, mu, att, att_global, idx, n, W, st);
-    case 6: return launch_field_h<R, 6>(g, tb, proj, mu, att, att_global, idx, n, W, st);
-    case 8: return launch_field_h<R, 8>(g, tb, proj, mu, att, att_global, idx, n, W, st);
-    case 10: return launch_field_h<R, 10>(g, tb, proj, mu, att, att_global, idx, n, W, st);
-    case 12: return launch_field_h<R, 12>(g, tb, proj, mu, att, att_global, idx, n, W, st);
-    case 14: return launch_field_h<R, 14>(g, tb, proj, mu, att, att_global, idx, n, W, st);
-    case 16: return launch_field_h<R, 16>(g, tb, proj, mu, att, att_global, idx, n, W, st);
-    case 18: return launch_field_h<R, 18>(g, tb, proj, mu, att, att_global, idx, n, W, st);
-    case 20: return launch_field_h<R, 20>(g, tb, proj, mu, att, att_global, idx, n, W, st);
+    SPECT_FIELD_CASE(0) SPECT_FIELD_CASE(2) SPECT_FIELD_CASE(4) SPECT_FIELD_CASE(6)
+    SPECT_FIELD_CASE(8) SPECT_FIELD_CASE(10) SPECT_FIELD_CASE(12) SPECT_FIELD_CASE(14)
+    SPECT_FIELD_CASE(16) SPECT_FIELD_CASE(18) SPECT_FIELD_CASE(20)
     default: return hipErrorInvalidValue;
   }
+#undef SPECT_FIELD_CASE
+}
+
+template <typename R>
+hipError_t launch_chunk_prefix(const Geo<R> &g, double *Dsum, int n, hipStream_t st) {
+  const size_t plane = (size_t)(g.je1 - g.je0) * g.Nz;
+  const size_t total = (size_t)n * plane;
+  chunk_prefix_kernel<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(Dsum, n, g.C, plane);
+  return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
                             const int32_t *idx, int n, int mode, int out_mode, R *out, const R *f,
-                            const R *nrm, R delta, R floor_value, hipStream_t st) {
+                            const R *nrm, R delta, R floor_value, const double *Wm,
+                            const double *Pref, R *out_mu, hipStream_t st) {
   constexpr int ZV = sizeof(R) == 4 ? 4 : 2;
   const size_t threads = (size_t)g.N * g.N * ((g.Nz + ZV - 1) / ZV);
   const unsigned blocks = (unsigned)((threads + 255) / 256);
   if (mode == SPECT_BP_REFERENCE)
     bp_gather_kernel<R, ZV, 0><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm,
-                                                       delta, floor_value);
-  else
+                                                       delta, floor_value, nullptr, nullptr, nullptr);
+  else if (mode == SPECT_BP_ADJOINT)
     bp_gather_kernel<R, ZV, 1><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm,
-                                                       delta, floor_value);
+                                                       delta, floor_value, nullptr, nullptr, nullptr);
+  else
+    bp_gather_kernel<R, ZV, 2><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm,
+                                                       delta, floor_value, Wm, Pref, out_mu);
   return hipGetLastError();
 }
 
 #define SPECT_INST(R)                                                                          \
   template hipError_t launch_bp_field<R>(const Geo<R> &, const Tables<R> &, const R *,         \
                                          const R *, const double *, bool, const int32_t *, int, \
-                                         R *, hipStream_t);                                    \
+                                         R *, const R *, double *, double *, hipStream_t);     \
+  template hipError_t launch_chunk_prefix<R>(const Geo<R> &, double *, int, hipStream_t);      \
   template hipError_t launch_bp_gather<R>(const Geo<R> &, const Tables<R> &, const R *,        \
                                           const int32_t *, int, int, int, R *, const R *,      \
-                                          const R *, R, R, hipStream_t);
+                                          const R *, R, R, const double *, const double *, R *, \
+                                          hipStream_t);
 SPECT_INST(float)
 SPECT_INST(double)
 #undef SPECT_INST

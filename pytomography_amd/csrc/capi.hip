@@ -144,6 +144,24 @@ Ws ws_layout(const spect_plan *p, int dtype, int n) {
   return l;
 }
 
+// Gradient calls: attenuation rows + W + Wm (FP64) + chunk totals (FP64).
+struct GradWs {
+  size_t att_off, w_off, wm_off, dsum_off, total;
+};
+
+GradWs grad_ws_layout(const spect_plan *p, int dtype, int n) {
+  const size_t rs = dtype == SPECT_F64 ? 8 : 4;
+  const Geo<float> g = geo<float>(p);
+  const size_t n_jext = g.je1 - g.je0;
+  GradWs l;
+  l.att_off = 0;
+  l.w_off = align256((size_t)n * p->C * n_jext * p->Nz * sizeof(double));
+  l.wm_off = l.w_off + align256((size_t)n * p->P * n_jext * p->Nz * rs);
+  l.dsum_off = l.wm_off + align256((size_t)n * p->P * n_jext * p->Nz * sizeof(double));
+  l.total = l.dsum_off + align256((size_t)n * p->C * n_jext * p->Nz * sizeof(double));
+  return l;
+}
+
 bool aligned16(const void *ptr) { return ((uintptr_t)ptr & 15) == 0; }
 
 int check_call(const spect_plan *p, int dtype, int n, const void *ws, size_t ws_bytes) {
@@ -211,9 +229,38 @@ int backward_impl(const spect_plan *p, int mode, int out_mode, const R *proj, co
   int rc = att_rows<R>(p, g, mu, table, idx, n, ws, l, st, &att, &global);
   if (rc) return rc;
   R *W = reinterpret_cast<R *>(ws + l.w_off);
-  HIP_TRY(launch_bp_field<R>(g, tables<R>(p), proj, mu, att, global, idx, n, W, st));
+  HIP_TRY(launch_bp_field<R>(g, tables<R>(p), proj, mu, att, global, idx, n, W, nullptr, nullptr,
+                             nullptr, st));
   HIP_TRY(launch_bp_gather<R>(g, tables<R>(p), W, idx, n, mode, out_mode, out, f, nrm, R(delta),
-                              R(floor_value), st));
+                              R(floor_value), nullptr, nullptr, nullptr, st));
+  return SPECT_OK;
+}
+
+template <typename R>
+int grad_impl(const spect_plan *p, const R *u, const R *f, const R *mu, const int32_t *idx, int n,
+              R *grad_f, R *grad_mu, unsigned char *ws, hipStream_t st) {
+  const Geo<R> g = geo<R>(p);
+  const GradWs l = grad_ws_layout(p, sizeof(R) == 8 ? SPECT_F64 : SPECT_F32, n);
+  R *W = reinterpret_cast<R *>(ws + l.w_off);
+  const double *att = nullptr;
+  if (mu) {
+    double *rows = reinterpret_cast<double *>(ws + l.att_off);
+    HIP_TRY(launch_att_table<R>(g, tables<R>(p), mu, idx, n, rows, st));
+    att = rows;
+  }
+  if (grad_mu) {
+    double *Wm = reinterpret_cast<double *>(ws + l.wm_off);
+    double *Dsum = reinterpret_cast<double *>(ws + l.dsum_off);
+    HIP_TRY(launch_bp_field<R>(g, tables<R>(p), u, mu, att, false, idx, n, W, f, Wm, Dsum, st));
+    HIP_TRY(launch_chunk_prefix<R>(g, Dsum, n, st));
+    HIP_TRY(launch_bp_gather<R>(g, tables<R>(p), W, idx, n, 2, SPECT_OUT_STORE, grad_f, nullptr,
+                                nullptr, R(0), R(0), Wm, Dsum, grad_mu, st));
+  } else {
+    HIP_TRY(launch_bp_field<R>(g, tables<R>(p), u, mu, att, false, idx, n, W, nullptr, nullptr,
+                               nullptr, st));
+    HIP_TRY(launch_bp_gather<R>(g, tables<R>(p), W, idx, n, SPECT_BP_ADJOINT, SPECT_OUT_STORE,
+                                grad_f, nullptr, nullptr, R(0), R(0), nullptr, nullptr, nullptr, st));
+  }
   return SPECT_OK;
 }
 
@@ -433,6 +480,33 @@ int spect_osem_subiteration(const spect_plan *p, int dtype, void *f, const void 
   return backward_impl<double>(p, SPECT_BP_REFERENCE, SPECT_OUT_OSEM, ratio, (const double *)mu,
                                att_table, angle_idx, n, (double *)f, (const double *)f,
                                (const double *)norm, delta, floor_value, ws, st);
+}
+
+size_t spect_grad_workspace_bytes(const spect_plan *p, int dtype, int n) {
+  if (!p || n <= 0) return 0;
+  return grad_ws_layout(p, dtype, n).total;
+}
+
+int spect_forward_vjp(const spect_plan *p, int dtype, const void *u, const void *f, const void *mu,
+                      const int32_t *angle_idx, int n, void *grad_f, void *grad_mu,
+                      void *workspace, size_t workspace_bytes, void *stream) {
+  if (!p) return fail(SPECT_E_ARG, "null plan");
+  if (dtype != SPECT_F32 && dtype != SPECT_F64) return fail(SPECT_E_ARG, "dtype must be F32 or F64");
+  if (dtype == SPECT_F64 && !p->has64) return fail(SPECT_E_ARG, "plan has no float64 tables");
+  if (n <= 0 || n > p->A) return fail(SPECT_E_ARG, "angle count out of range");
+  if (!u || !angle_idx || !grad_f) return fail(SPECT_E_ARG, "null array argument");
+  if (grad_mu && (!mu || !f)) return fail(SPECT_E_ARG, "the attenuation gradient needs mu and f");
+  const size_t need = grad_ws_layout(p, dtype, n).total;
+  if (!workspace || workspace_bytes < need)
+    return fail(SPECT_E_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  if (int rc = check_arrays({u, f, mu, grad_f, grad_mu, workspace})) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  auto *ws = reinterpret_cast<unsigned char *>(workspace);
+  if (dtype == SPECT_F32)
+    return grad_impl<float>(p, (const float *)u, (const float *)f, (const float *)mu, angle_idx, n,
+                            (float *)grad_f, (float *)grad_mu, ws, st);
+  return grad_impl<double>(p, (const double *)u, (const double *)f, (const double *)mu, angle_idx, n,
+                           (double *)grad_f, (double *)grad_mu, ws, st);
 }
 
 int spect_poisson_ratio(int dtype, const void *g, const void *hf, const void *s, void *pred,

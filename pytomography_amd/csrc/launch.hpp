@@ -33,18 +33,26 @@ hipError_t launch_forward_reduce(const Geo<R> &g, const R *partial, int n, R *pr
                                  const R *g_meas, const R *s, R *pred, R *ratio, R delta,
                                  hipStream_t st);
 
-// Back projection field W[n][P][je1-je0][Nz] = a * (G_i * p).
+// Back projection field W[n][P][je1-je0][Nz] = a * (G_i * p).  With Wm non-null
+// also the attenuation-gradient field (needs f): Wm[n][P][je1-je0][Nz] and the
+// chunk totals Dsum[n][C][je1-je0][Nz], both FP64.
 template <typename R>
 hipError_t launch_bp_field(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
                            const double *att, bool att_global, const int32_t *angle_idx, int n,
-                           R *W, hipStream_t st);
+                           R *W, const R *f, double *Wm, double *Dsum, hipStream_t st);
 
-// Gather W back into the object (mode: 0 reference rotation, 1 exact adjoint)
-// with out_mode 0 store, 1 add, 2 OSEM update.
+// Dsum <- prefix over chunks (in place).
+template <typename R>
+hipError_t launch_chunk_prefix(const Geo<R> &g, double *Dsum, int n, hipStream_t st);
+
+// Gather W back into the object: mode 0 reference rotation, 1 exact adjoint,
+// 2 exact adjoint of W -> out and of dx*(Wm - Pref) -> out_mu; out_mode 0
+// store, 1 add, 2 OSEM update.
 template <typename R>
 hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
                             const int32_t *angle_idx, int n, int mode, int out_mode, R *out,
-                            const R *f, const R *nrm, R delta, R floor_value, hipStream_t st);
+                            const R *f, const R *nrm, R delta, R floor_value, const double *Wm,
+                            const double *Pref, R *out_mu, hipStream_t st);
 
 // Elementwise and stencil kernels (misc.hip).
 template <typename R>

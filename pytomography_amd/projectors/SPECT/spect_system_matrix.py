@@ -327,6 +327,51 @@ class SPECTSystemMatrix(SystemMatrix):
             float(floor_value), ws.data_ptr(), nbytes, _lib.stream_ptr(device)))
         return pred_out
 
+    # ------------------------------------------------- differentiable projector
+    def forward_with_mu(self, object: torch.Tensor, mu: torch.Tensor | None,
+                        subset_idx: int | None = None) -> torch.Tensor:
+        """H(mu) f with an explicit attenuation map (rows rebuilt every call, no
+        cache): the forward of the differentiable projector."""
+        device = object.device
+        dt = self._compute_dtype(object)
+        f = _lib.require_device_tensor(object, "object", dt)
+        m = None if mu is None else _lib.require_device_tensor(mu.detach(), "mu", dt)
+        if (m is None) != (self._att is None):
+            raise ValueError("mu must be given exactly when the system matrix models attenuation")
+        plan = self._plan(device)
+        idx = self._angle_index(subset_idx, device)
+        n = idx.numel()
+        out = torch.empty((n, self._shape[1], self._shape[2]), dtype=dt, device=device)
+        ws, nbytes = self._workspace(device, dt, n)
+        _lib.check(plan.lib.spect_forward(plan.handle, _lib.dtype_code(dt), f.data_ptr(), _lib.ptr(m),
+                                          None, idx.data_ptr(), n, out.data_ptr(), ws.data_ptr(),
+                                          nbytes, _lib.stream_ptr(device)))
+        return out
+
+    def forward_vjp(self, grad_proj: torch.Tensor, object: torch.Tensor, mu: torch.Tensor | None,
+                    subset_idx: int | None = None, need_mu: bool = True):
+        """(H(mu)^T u, (d H(mu) f / d mu)^T u) for upstream gradient u = grad_proj."""
+        device = object.device
+        dt = self._compute_dtype(object)
+        u = _lib.require_device_tensor(grad_proj, "grad_proj", dt)
+        f = _lib.require_device_tensor(object.detach(), "object", dt)
+        m = None if mu is None else _lib.require_device_tensor(mu.detach(), "mu", dt)
+        plan = self._plan(device)
+        idx = self._angle_index(subset_idx, device)
+        n = idx.numel()
+        gf = torch.empty(self._shape, dtype=dt, device=device)
+        gm = torch.empty(self._shape, dtype=dt, device=device) if (need_mu and m is not None) else None
+        nbytes = int(plan.lib.spect_grad_workspace_bytes(plan.handle, _lib.dtype_code(dt), n))
+        key = ("grad", device.type, device.index)
+        ws = self._ws.get(key)
+        if ws is None or ws.numel() < nbytes:
+            self._ws[key] = ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        _lib.check(plan.lib.spect_forward_vjp(plan.handle, _lib.dtype_code(dt), u.data_ptr(),
+                                              f.data_ptr(), _lib.ptr(m), idx.data_ptr(), n,
+                                              gf.data_ptr(), _lib.ptr(gm), ws.data_ptr(), ws.numel(),
+                                              _lib.stream_ptr(device)))
+        return gf, gm
+
     def _get_object_initial(self, device=None):
         """Ones (spect_system_matrix.py:38-44)."""
         if device is None:
