@@ -154,6 +154,38 @@ int spect_forward_vjp(const spect_plan *p, int dtype, const void *u, const void 
                       const int32_t *angle_idx, int n, void *grad_f, void *grad_mu,
                       void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Batched OSEM over many independent problems (no counterpart in the
+ * reference, which runs one reconstruction at a time through the Python loop
+ * of preconditioned_gradient_ascent.py:103-122).  Every sub-iteration of every
+ * problem is the spect_osem_subiteration of that problem; the batch launches
+ * each kernel once per sub-iteration for all problems that share the PSF
+ * width and z extent, and can capture the whole schedule in a hipGraph. */
+typedef struct spect_batch spect_batch;
+
+typedef struct spect_batch_problem {
+  const spect_plan *plan;
+  void *f;                          /* [nx][ny][nz], updated in place */
+  const void *mu;                   /* attenuation map or NULL */
+  const void *att_table;            /* spect_att_table_build(mu) result; required with mu */
+  int32_t n_subsets;                /* equal for all problems of a batch */
+  const int32_t *const *angle_idx;  /* host [n_subsets] of device int32 [count] */
+  const int32_t *counts;            /* host [n_subsets] */
+  const void *const *g;             /* host [n_subsets] of device [count][ny][nz] */
+  const void *const *s;             /* host [n_subsets] of device [count][ny][nz] or NULL */
+  const void *const *norm;          /* host [n_subsets] of device [nx][ny][nz] */
+  void *const *pred;                /* host [n_subsets] of device [count][ny][nz] or NULL */
+  void *workspace;                  /* >= spect_workspace_bytes(plan, dtype, max count) */
+  size_t workspace_bytes;
+} spect_batch_problem;
+
+int spect_batch_create(const spect_batch_problem *probs, int n, int dtype, double delta,
+                       double floor_value, spect_batch **out);
+/* n_iters full iterations (all subsets, all problems).  use_graph != 0 captures
+ * the schedule into a hipGraph on the first call (per n_iters) and replays it. */
+int spect_batch_run(spect_batch *b, int n_iters, int use_graph, void *stream);
+int spect_batch_destroy(spect_batch *b);
+
 /* Poisson ratio of PoissonLogLikelihood.compute_gradient
  * (likelihoods/poisson_log_likelihood.py:34-36):
  *   pred = hf + s (s may be NULL);  ratio = g / (pred + delta).

@@ -24,6 +24,7 @@ EXPORTED = (
     "spect_plan_dims", "spect_workspace_bytes", "spect_att_table_bytes", "spect_att_table_build",
     "spect_forward", "spect_backward", "spect_osem_subiteration",
     "spect_grad_workspace_bytes", "spect_forward_vjp",
+    "spect_batch_create", "spect_batch_run", "spect_batch_destroy",
     "spect_poisson_ratio", "spect_rdp_gradient", "spect_bsrem_update",
 )
 
@@ -47,6 +48,17 @@ class SpectGeom(ctypes.Structure):
         ("taps_r64", ctypes.POINTER(ctypes.c_double)),
         ("taps_z64", ctypes.POINTER(ctypes.c_double)),
         ("chunk_planes", ctypes.c_int32),
+    ]
+
+
+class SpectBatchProblem(ctypes.Structure):
+    _fields_ = [
+        ("plan", ctypes.c_void_p), ("f", ctypes.c_void_p), ("mu", ctypes.c_void_p),
+        ("att_table", ctypes.c_void_p), ("n_subsets", ctypes.c_int32),
+        ("angle_idx", ctypes.POINTER(ctypes.c_void_p)), ("counts", ctypes.POINTER(ctypes.c_int32)),
+        ("g", ctypes.POINTER(ctypes.c_void_p)), ("s", ctypes.POINTER(ctypes.c_void_p)),
+        ("norm", ctypes.POINTER(ctypes.c_void_p)), ("pred", ctypes.POINTER(ctypes.c_void_p)),
+        ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t),
     ]
 
 
@@ -82,6 +94,13 @@ def _declare(lib):
     lib.spect_grad_workspace_bytes.argtypes = [vp, ci, ci]
     lib.spect_forward_vjp.restype = ci
     lib.spect_forward_vjp.argtypes = [vp, ci, vp, vp, vp, vp, ci, vp, vp, vp, sz, vp]
+    lib.spect_batch_create.restype = ci
+    lib.spect_batch_create.argtypes = [ctypes.POINTER(SpectBatchProblem), ci, ci, dbl, dbl,
+                                       ctypes.POINTER(vp)]
+    lib.spect_batch_run.restype = ci
+    lib.spect_batch_run.argtypes = [vp, ci, ci, vp]
+    lib.spect_batch_destroy.restype = ci
+    lib.spect_batch_destroy.argtypes = [vp]
     lib.spect_poisson_ratio.restype = ci
     lib.spect_poisson_ratio.argtypes = [ci, vp, vp, vp, vp, vp, i64, dbl, vp]
     lib.spect_rdp_gradient.restype = ci

@@ -32,21 +32,22 @@ namespace {
 // chunk-local part Wm(k) = sum_{i>k, i in chunk} D(i) + D(k)/2 (FP64) and the
 // chunk total Dsum (FP64); the gather subtracts the prefix of chunk totals.
 template <typename R, int RZ, int QS, int H, bool GRAD>
-__global__ __launch_bounds__(1024) void bp_field_kernel(Geo<R> g, Tables<R> tb,
-                                                       const R *__restrict__ proj,
-                                                       const R *__restrict__ mu,
-                                                       const double *__restrict__ att,
-                                                       int att_global,
-                                                       const int32_t *__restrict__ angle_idx,
-                                                       R *__restrict__ W,
-                                                       const R *__restrict__ f,
-                                                       double *__restrict__ Wm,
-                                                       double *__restrict__ Dsum) {
+__device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt, int c,
+                                           unsigned char *smem) {
+  const Geo<R> &g = J.g;
+  const Tables<R> &tb = J.tb;
+  const R *__restrict__ proj = J.proj;
+  const R *__restrict__ mu = J.mu;
+  const double *__restrict__ att = J.att;
+  const int att_global = J.att_global;
+  const int32_t *__restrict__ angle_idx = J.angle_idx;
+  R *__restrict__ W = J.W;
+  const R *__restrict__ f = J.f;
+  double *__restrict__ Wm = J.Wm;
+  double *__restrict__ Dsum = J.Dsum;
   constexpr int K = 2 * H + 1;
   constexpr int HALF_ROWS = kTJ / QS;  // tile rows per y-phase thread
-  extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Src<R> src[2][kTJ];
-  const int al = blockIdx.x, jt = blockIdx.y, c = blockIdx.z;
   const int ang = ld_uniform(angle_idx + al);
   if ((unsigned)ang >= (unsigned)g.A) return;
   const int tid = threadIdx.x;
@@ -203,6 +204,25 @@ __global__ __launch_bounds__(1024) void bp_field_kernel(Geo<R> g, Tables<R> tb,
   }
 }
 
+template <typename R, int RZ, int QS, int H, bool GRAD>
+__global__ __launch_bounds__(1024) void bp_field_kernel(FieldJob<R> J) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  field_body<R, RZ, QS, H, GRAD>(J, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+}
+
+template <typename R, int RZ, int QS, int H>
+__global__ __launch_bounds__(1024) void field_batch_kernel(const FieldJob<R> *__restrict__ jobs,
+                                                           const int *__restrict__ starts,
+                                                           int njobs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = blockIdx.x;
+  const int j = find_job(starts, njobs, b);
+  const FieldJob<R> J = ld_uniform_struct(jobs + j);
+  const int local = b - ld_uniform(starts + j);
+  const int c = local % J.g.C, r = local / J.g.C;
+  field_body<R, RZ, QS, H, false>(J, r / J.g.ntiles, r % J.g.ntiles, c, smem);
+}
+
 // In place: Dsum[a][c] <- sum_{c' <= c} Dsum[a][c'] (prefix over chunks, FP64).
 __global__ __launch_bounds__(256) void chunk_prefix_kernel(double *__restrict__ Dsum, int n, int C,
                                                            size_t plane) {
@@ -275,20 +295,22 @@ __device__ __forceinline__ void bp_store_vec(int out_mode, R *__restrict__ out, 
 // mode 2: exact transpose as mode 1 for W -> out, and for the attenuation
 // gradient field dx * (Wm - prefix of chunk totals) -> out_mu (FP64 sums).
 template <typename R, int ZV, int MODE>
-__global__ __launch_bounds__(256) void bp_gather_kernel(Geo<R> g, Tables<R> tb,
-                                                        const R *__restrict__ W,
-                                                        const int32_t *__restrict__ angle_idx,
-                                                        int n_call, int out_mode,
-                                                        R *__restrict__ out,
-                                                        const R *__restrict__ f,
-                                                        const R *__restrict__ nrm, R delta,
-                                                        R floor_value,
-                                                        const double *__restrict__ Wm,
-                                                        const double *__restrict__ Pref,
-                                                        R *__restrict__ out_mu) {
+__device__ __forceinline__ void gather_body(const GatherJob<R> &J, size_t gid) {
+  const Geo<R> &g = J.g;
+  const Tables<R> &tb = J.tb;
+  const R *__restrict__ W = J.W;
+  const int32_t *__restrict__ angle_idx = J.angle_idx;
+  const int n_call = J.n;
+  const int out_mode = J.out_mode;
+  R *__restrict__ out = J.out;
+  const R *__restrict__ f = J.f;
+  const R *__restrict__ nrm = J.nrm;
+  const R delta = J.delta, floor_value = J.floor_value;
+  const double *__restrict__ Wm = J.Wm;
+  const double *__restrict__ Pref = J.Pref;
+  R *__restrict__ out_mu = J.out_mu;
   const int Nz = g.Nz, N = g.N;
   const int zq = (Nz + ZV - 1) / ZV;  // z groups per column
-  const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (gid >= (size_t)N * N * zq) return;
   const int col = (int)(gid / zq);
   const int z0 = (int)(gid - (size_t)col * zq) * ZV;
@@ -404,6 +426,21 @@ __global__ __launch_bounds__(256) void bp_gather_kernel(Geo<R> g, Tables<R> tb,
   }
 }
 
+template <typename R, int ZV, int MODE>
+__global__ __launch_bounds__(256) void bp_gather_kernel(GatherJob<R> J) {
+  gather_body<R, ZV, MODE>(J, (size_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+template <typename R, int ZV, int MODE>
+__global__ __launch_bounds__(256) void gather_batch_kernel(const GatherJob<R> *__restrict__ jobs,
+                                                           const int *__restrict__ starts,
+                                                           int njobs) {
+  const int b = blockIdx.x;
+  const int j = find_job(starts, njobs, b);
+  const GatherJob<R> J = ld_uniform_struct(jobs + j);
+  gather_body<R, ZV, MODE>(J, (size_t)(b - ld_uniform(starts + j)) * 256 + threadIdx.x);
+}
+
 template <typename R, int H>
 hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
                           const double *att, bool att_global, const int32_t *idx, int n, R *W,
@@ -420,7 +457,7 @@ hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, c
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
       if (e != hipSuccess) return e;
     }
-    kern<<<grid, bt, smem, st>>>(g, tb, proj, mu, att, att_global ? 1 : 0, idx, W, f, Wm, Dsum);
+    kern<<<grid, bt, smem, st>>>(FieldJob<R>{g, tb, proj, mu, att, att_global ? 1 : 0, idx, n, W, f, Wm, Dsum});
   } else {
     auto kern = bp_field_kernel<R, RZ, QS, H, false>;
     if (smem > 64 * 1024) {
@@ -428,8 +465,8 @@ hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, c
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
       if (e != hipSuccess) return e;
     }
-    kern<<<grid, bt, smem, st>>>(g, tb, proj, mu, att, att_global ? 1 : 0, idx, W, nullptr,
-                                 nullptr, nullptr);
+    kern<<<grid, bt, smem, st>>>(
+        FieldJob<R>{g, tb, proj, mu, att, att_global ? 1 : 0, idx, n, W, nullptr, nullptr, nullptr});
   }
   return hipGetLastError();
 }
@@ -465,17 +502,56 @@ hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
                             const R *nrm, R delta, R floor_value, const double *Wm,
                             const double *Pref, R *out_mu, hipStream_t st) {
   constexpr int ZV = sizeof(R) == 4 ? 4 : 2;
-  const size_t threads = (size_t)g.N * g.N * ((g.Nz + ZV - 1) / ZV);
-  const unsigned blocks = (unsigned)((threads + 255) / 256);
+  const unsigned blocks = (unsigned)gather_blocks(g);
+  const GatherJob<R> J{g, tb, W, idx, n, out_mode, out, f, nrm, delta, floor_value, Wm, Pref, out_mu};
   if (mode == SPECT_BP_REFERENCE)
-    bp_gather_kernel<R, ZV, 0><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm,
-                                                       delta, floor_value, nullptr, nullptr, nullptr);
+    bp_gather_kernel<R, ZV, 0><<<blocks, 256, 0, st>>>(J);
   else if (mode == SPECT_BP_ADJOINT)
-    bp_gather_kernel<R, ZV, 1><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm,
-                                                       delta, floor_value, nullptr, nullptr, nullptr);
+    bp_gather_kernel<R, ZV, 1><<<blocks, 256, 0, st>>>(J);
   else
-    bp_gather_kernel<R, ZV, 2><<<blocks, 256, 0, st>>>(g, tb, W, idx, n, out_mode, out, f, nrm,
-                                                       delta, floor_value, Wm, Pref, out_mu);
+    bp_gather_kernel<R, ZV, 2><<<blocks, 256, 0, st>>>(J);
+  return hipGetLastError();
+}
+
+template <typename R, int H>
+hipError_t launch_field_batch_h(const FieldJob<R> *jobs, const int *starts, int njobs, int blocks,
+                                int Nz, hipStream_t st) {
+  constexpr int RZ = BpTile<R>::RZ, QS = BpTile<R>::QS;
+  const int bt = tile_block_threads(Nz, RZ, QS);
+  const int runs = (Nz + RZ - 1) / RZ;
+  const size_t smem = (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
+  auto kern = field_batch_kernel<R, RZ, QS, H>;
+  if (smem > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+  }
+  kern<<<blocks, bt, smem, st>>>(jobs, starts, njobs);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_field_batch(const FieldJob<R> *jobs, const int *starts, int njobs, int blocks,
+                              int H, int Nz, hipStream_t st) {
+#define SPECT_FLB_CASE(HH) \
+  case HH: return launch_field_batch_h<R, HH>(jobs, starts, njobs, blocks, Nz, st);
+  switch (H) {
+    SPECT_FLB_CASE(0) SPECT_FLB_CASE(2) SPECT_FLB_CASE(4) SPECT_FLB_CASE(6) SPECT_FLB_CASE(8)
+    SPECT_FLB_CASE(10) SPECT_FLB_CASE(12) SPECT_FLB_CASE(14) SPECT_FLB_CASE(16)
+    SPECT_FLB_CASE(18) SPECT_FLB_CASE(20)
+    default: return hipErrorInvalidValue;
+  }
+#undef SPECT_FLB_CASE
+}
+
+template <typename R>
+hipError_t launch_gather_batch(const GatherJob<R> *jobs, const int *starts, int njobs, int blocks,
+                               int mode, hipStream_t st) {
+  constexpr int ZV = sizeof(R) == 4 ? 4 : 2;
+  if (mode == SPECT_BP_REFERENCE)
+    gather_batch_kernel<R, ZV, 0><<<blocks, 256, 0, st>>>(jobs, starts, njobs);
+  else
+    gather_batch_kernel<R, ZV, 1><<<blocks, 256, 0, st>>>(jobs, starts, njobs);
   return hipGetLastError();
 }
 
@@ -484,6 +560,10 @@ hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
                                          const R *, const double *, bool, const int32_t *, int, \
                                          R *, const R *, double *, double *, hipStream_t);     \
   template hipError_t launch_chunk_prefix<R>(const Geo<R> &, double *, int, hipStream_t);      \
+  template hipError_t launch_field_batch<R>(const FieldJob<R> *, const int *, int, int, int,    \
+                                            int, hipStream_t);                                 \
+  template hipError_t launch_gather_batch<R>(const GatherJob<R> *, const int *, int, int, int,  \
+                                             hipStream_t);                                     \
   template hipError_t launch_bp_gather<R>(const Geo<R> &, const Tables<R> &, const R *,        \
                                           const int32_t *, int, int, int, R *, const R *,      \
                                           const R *, R, R, const double *, const double *, R *, \

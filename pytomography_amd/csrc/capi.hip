@@ -266,6 +266,223 @@ int grad_impl(const spect_plan *p, const R *u, const R *f, const R *mu, const in
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Batched OSEM
+// ---------------------------------------------------------------------------
+struct spect_batch {
+  int dtype;
+  int n_subsets;
+  struct Group {
+    int H, Nz;
+    int nprob;
+    // per subset: device offsets (bytes) into `tables` and block counts
+    std::vector<size_t> fwd_off, red_off, fld_off, gat_off, starts_off[4];
+    std::vector<int> blocks[4];
+  };
+  std::vector<Group> groups;
+  void *tables = nullptr;  // device: job arrays and block starts
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;  // (n_iters, exec)
+  hipStream_t capture_stream = nullptr;  // private stream the schedule is captured on
+};
+
+namespace {
+
+template <typename R>
+int batch_build(spect_batch *b, const spect_batch_problem *probs, int n, double delta,
+                double floor_value) {
+  // group problems by (H, Nz): a launch compiles one PSF width and one block shape
+  std::vector<std::vector<int>> members;
+  std::vector<std::pair<int, int>> keys;
+  for (int i = 0; i < n; ++i) {
+    const std::pair<int, int> key{probs[i].plan->H, probs[i].plan->Nz};
+    size_t k = 0;
+    while (k < keys.size() && keys[k] != key) ++k;
+    if (k == keys.size()) {
+      keys.push_back(key);
+      members.emplace_back();
+    }
+    members[k].push_back(i);
+  }
+  const int S = b->n_subsets;
+  std::vector<unsigned char> host;
+  auto put = [&](const void *src, size_t bytes) {
+    const size_t off = (host.size() + 255) & ~size_t(255);
+    host.resize(off + bytes);
+    std::memcpy(host.data() + off, src, bytes);
+    return off;
+  };
+  for (size_t k = 0; k < keys.size(); ++k) {
+    spect_batch::Group G;
+    G.H = keys[k].first;
+    G.Nz = keys[k].second;
+    G.nprob = (int)members[k].size();
+    for (int sub = 0; sub < S; ++sub) {
+      std::vector<FwdJob<R>> fj;
+      std::vector<ReduceJob<R>> rj;
+      std::vector<FieldJob<R>> lj;
+      std::vector<GatherJob<R>> gj;
+      std::vector<int> st[4];
+      int tot[4] = {0, 0, 0, 0};
+      for (int i : members[k]) {
+        const spect_batch_problem &pb = probs[i];
+        const spect_plan *p = pb.plan;
+        const Geo<R> g = geo<R>(p);
+        const Tables<R> tb = tables<R>(p);
+        const int cnt = pb.counts[sub];
+        const Ws l = ws_layout(p, sizeof(R) == 8 ? SPECT_F64 : SPECT_F32, cnt);
+        auto *ws = reinterpret_cast<unsigned char *>(pb.workspace);
+        R *part = reinterpret_cast<R *>(ws + l.part_off);
+        R *W = reinterpret_cast<R *>(ws + l.w_off);
+        R *ratio = reinterpret_cast<R *>(ws + l.ratio_off);
+        const R *mu = reinterpret_cast<const R *>(pb.mu);
+        const double *att = reinterpret_cast<const double *>(pb.att_table);
+        const int32_t *idx = pb.angle_idx[sub];
+        R *f = reinterpret_cast<R *>(pb.f);
+        fj.push_back(FwdJob<R>{g, tb, f, mu, att, 1, idx, cnt, part});
+        rj.push_back(ReduceJob<R>{g, part, cnt, nullptr, reinterpret_cast<const R *>(pb.g[sub]),
+                                  pb.s ? reinterpret_cast<const R *>(pb.s[sub]) : nullptr,
+                                  pb.pred ? reinterpret_cast<R *>(pb.pred[sub]) : nullptr, ratio,
+                                  R(delta)});
+        lj.push_back(FieldJob<R>{g, tb, ratio, mu, att, 1, idx, cnt, W, nullptr, nullptr, nullptr});
+        gj.push_back(GatherJob<R>{g, tb, W, idx, cnt, SPECT_OUT_OSEM, f, f,
+                                  reinterpret_cast<const R *>(pb.norm[sub]), R(delta),
+                                  R(floor_value), nullptr, nullptr, nullptr});
+        const int nb[4] = {fwd_blocks(g, cnt), reduce_blocks(g, cnt), fwd_blocks(g, cnt),
+                           gather_blocks(g)};
+        for (int q = 0; q < 4; ++q) {
+          st[q].push_back(tot[q]);
+          tot[q] += nb[q];
+        }
+      }
+      G.fwd_off.push_back(put(fj.data(), fj.size() * sizeof(fj[0])));
+      G.red_off.push_back(put(rj.data(), rj.size() * sizeof(rj[0])));
+      G.fld_off.push_back(put(lj.data(), lj.size() * sizeof(lj[0])));
+      G.gat_off.push_back(put(gj.data(), gj.size() * sizeof(gj[0])));
+      for (int q = 0; q < 4; ++q) {
+        G.starts_off[q].push_back(put(st[q].data(), st[q].size() * sizeof(int)));
+        G.blocks[q].push_back(tot[q]);
+      }
+    }
+    b->groups.push_back(G);
+  }
+  HIP_TRY(hipMalloc(&b->tables, host.size()));
+  HIP_TRY(hipMemcpy(b->tables, host.data(), host.size(), hipMemcpyHostToDevice));
+  return SPECT_OK;
+}
+
+template <typename R>
+int batch_launch(spect_batch *b, int n_iters, hipStream_t st) {
+  auto *base = reinterpret_cast<unsigned char *>(b->tables);
+  for (int it = 0; it < n_iters; ++it) {
+    for (int sub = 0; sub < b->n_subsets; ++sub) {
+      for (const auto &G : b->groups) {
+        HIP_TRY(launch_forward_batch<R>(reinterpret_cast<const FwdJob<R> *>(base + G.fwd_off[sub]),
+                                        reinterpret_cast<const int *>(base + G.starts_off[0][sub]),
+                                        G.nprob, G.blocks[0][sub], G.H, G.Nz, st));
+      }
+      for (const auto &G : b->groups) {
+        HIP_TRY(launch_reduce_batch<R>(reinterpret_cast<const ReduceJob<R> *>(base + G.red_off[sub]),
+                                       reinterpret_cast<const int *>(base + G.starts_off[1][sub]),
+                                       G.nprob, G.blocks[1][sub], st));
+      }
+      for (const auto &G : b->groups) {
+        HIP_TRY(launch_field_batch<R>(reinterpret_cast<const FieldJob<R> *>(base + G.fld_off[sub]),
+                                      reinterpret_cast<const int *>(base + G.starts_off[2][sub]),
+                                      G.nprob, G.blocks[2][sub], G.H, G.Nz, st));
+      }
+      for (const auto &G : b->groups) {
+        HIP_TRY(launch_gather_batch<R>(reinterpret_cast<const GatherJob<R> *>(base + G.gat_off[sub]),
+                                       reinterpret_cast<const int *>(base + G.starts_off[3][sub]),
+                                       G.nprob, G.blocks[3][sub], SPECT_BP_REFERENCE, st));
+      }
+    }
+  }
+  return SPECT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spect_batch_create(const spect_batch_problem *probs, int n, int dtype, double delta,
+                       double floor_value, spect_batch **out) {
+  if (!probs || n <= 0 || !out) return fail(SPECT_E_ARG, "bad batch arguments");
+  *out = nullptr;
+  if (dtype != SPECT_F32 && dtype != SPECT_F64) return fail(SPECT_E_ARG, "dtype must be F32 or F64");
+  const int S = probs[0].n_subsets;
+  for (int i = 0; i < n; ++i) {
+    const spect_batch_problem &pb = probs[i];
+    if (!pb.plan || !pb.f || !pb.angle_idx || !pb.counts || !pb.g || !pb.norm || !pb.workspace)
+      return fail(SPECT_E_ARG, "batch problem " + std::to_string(i) + ": null argument");
+    if (pb.n_subsets != S || S <= 0) return fail(SPECT_E_ARG, "all problems need the same subset count");
+    if (dtype == SPECT_F64 && !pb.plan->has64) return fail(SPECT_E_ARG, "plan has no float64 tables");
+    if (pb.mu && !pb.att_table) return fail(SPECT_E_ARG, "batched problems with mu need an attenuation table");
+    int cmax = 0;
+    for (int k = 0; k < S; ++k) {
+      if (pb.counts[k] <= 0 || pb.counts[k] > pb.plan->A || !pb.angle_idx[k] || !pb.g[k] || !pb.norm[k])
+        return fail(SPECT_E_ARG, "batch problem " + std::to_string(i) + ": bad subset " + std::to_string(k));
+      cmax = std::max(cmax, (int)pb.counts[k]);
+    }
+    if (pb.workspace_bytes < ws_layout(pb.plan, dtype, cmax).total)
+      return fail(SPECT_E_WORKSPACE, "batch problem " + std::to_string(i) + ": workspace too small");
+  }
+  auto *b = new spect_batch();
+  b->dtype = dtype;
+  b->n_subsets = S;
+  const int rc = dtype == SPECT_F32 ? batch_build<float>(b, probs, n, delta, floor_value)
+                                    : batch_build<double>(b, probs, n, delta, floor_value);
+  if (rc) {
+    spect_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return SPECT_OK;
+}
+
+int spect_batch_run(spect_batch *b, int n_iters, int use_graph, void *stream) {
+  if (!b || n_iters <= 0) return fail(SPECT_E_ARG, "bad batch run arguments");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (!use_graph)
+    return b->dtype == SPECT_F32 ? batch_launch<float>(b, n_iters, st) : batch_launch<double>(b, n_iters, st);
+  for (auto &e : b->graphs)
+    if (e.first == n_iters) {
+      HIP_TRY(hipGraphLaunch(e.second, st));
+      return SPECT_OK;
+    }
+  // Capture on a private stream (the caller's may be the null stream, which
+  // cannot be captured); the graph is then launched on the caller's stream.
+  if (!b->capture_stream) HIP_TRY(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+  hipStream_t cs = b->capture_stream;
+  HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  const int rc = b->dtype == SPECT_F32 ? batch_launch<float>(b, n_iters, cs)
+                                       : batch_launch<double>(b, n_iters, cs);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(cs, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ec != hipSuccess) return hip_fail(ec, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate");
+  b->graphs.emplace_back(n_iters, exec);
+  HIP_TRY(hipGraphLaunch(exec, st));
+  return SPECT_OK;
+}
+
+int spect_batch_destroy(spect_batch *b) {
+  if (!b) return SPECT_OK;
+  for (auto &e : b->graphs) (void)hipGraphExecDestroy(e.second);
+  if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  (void)hipFree(b->tables);
+  delete b;
+  return SPECT_OK;
+}
+
+}  // extern "C"
+
 extern "C" {
 
 const char *spect_last_error(void) { return g_err.c_str(); }

@@ -93,6 +93,28 @@ template <typename T>
 __device__ __forceinline__ T ld_uniform(const T *p) {
   return *(const __attribute__((address_space(4))) T *)(p);
 }
+// Whole descriptor struct (plain data) through scalar loads.
+template <typename T>
+__device__ __forceinline__ T ld_uniform_struct(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "descriptor size must be a multiple of 4");
+  union {
+    T t;
+    uint32_t w[sizeof(T) / 4];
+  } u;
+  const uint32_t *q = reinterpret_cast<const uint32_t *>(p);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) u.w[k] = ld_uniform(q + k);
+  return u.t;
+}
+
+// Index of the job owning block b: starts[] ascending, starts[0] == 0.
+__device__ __forceinline__ int find_job(const int *starts, int njobs, int b) {
+  int j = 0;
+  for (int k = 1; k < njobs; ++k)
+    if (ld_uniform(starts + k) <= b) j = k;
+  return j;
+}
+
 __device__ __forceinline__ float2 ld_uniform(const float2 *p) {
   const float *q = reinterpret_cast<const float *>(p);
   return make_float2(ld_uniform(q), ld_uniform(q + 1));

@@ -96,17 +96,21 @@ template <int RZ, int H> __device__ __forceinline__ int lds_stride(int runs) {
 }
 
 template <typename R, int RZ, int QS, int H>
-__global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const R *__restrict__ f,
-                                                  const R *__restrict__ mu,
-                                                  const double *__restrict__ att, int att_global,
-                                                  const int32_t *__restrict__ angle_idx,
-                                                  R *__restrict__ partial, int n_call) {
+__device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int c,
+                                         unsigned char *smem) {
   constexpr int K = 2 * H + 1;
   constexpr int JO = kTJ + 2 * H;      // output columns touched by the tile
   constexpr int JH = (JO + QS - 1) / QS;  // per split
-  extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Src<R> src[kTJ];
-  const int al = blockIdx.x, jt = blockIdx.y, c = blockIdx.z;
+  const Geo<R> &g = J.g;
+  const Tables<R> &tb = J.tb;
+  const R *__restrict__ f = J.f;
+  const R *__restrict__ mu = J.mu;
+  const double *__restrict__ att = J.att;
+  const int att_global = J.att_global;
+  const int32_t *__restrict__ angle_idx = J.angle_idx;
+  R *__restrict__ partial = J.partial;
+  const int n_call = J.n;
   const int ang = ld_uniform(angle_idx + al);
   if ((unsigned)ang >= (unsigned)g.A) return;
   const int tid = threadIdx.x;
@@ -269,16 +273,39 @@ __global__ __launch_bounds__(512) void fwd_kernel(Geo<R> g, Tables<R> tb, const 
   }
 }
 
+template <typename R, int RZ, int QS, int H>
+__global__ __launch_bounds__(512) void fwd_kernel(FwdJob<R> J) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  fwd_body<R, RZ, QS, H>(J, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+}
+
+// Batched: block b belongs to job find_job(b); within a job the blocks run
+// over (angle, tile, chunk) with the chunk fastest.
+template <typename R, int RZ, int QS, int H>
+__global__ __launch_bounds__(512) void fwd_batch_kernel(const FwdJob<R> *__restrict__ jobs,
+                                                        const int *__restrict__ starts, int njobs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = blockIdx.x;
+  const int j = find_job(starts, njobs, b);
+  const FwdJob<R> J = ld_uniform_struct(jobs + j);
+  const int local = b - ld_uniform(starts + j);
+  const int c = local % J.g.C, r = local / J.g.C;
+  fwd_body<R, RZ, QS, H>(J, r / J.g.ntiles, r % J.g.ntiles, c, smem);
+}
+
 template <typename R>
-__global__ __launch_bounds__(256) void fwd_reduce_kernel(Geo<R> g, const R *__restrict__ partial,
-                                                         int n, R *__restrict__ proj,
-                                                         const R *__restrict__ gm,
-                                                         const R *__restrict__ s,
-                                                         R *__restrict__ pred, R *__restrict__ ratio,
-                                                         R delta) {
+__device__ __forceinline__ void reduce_body(const ReduceJob<R> &J, size_t idx) {
+  const Geo<R> &g = J.g;
+  const R *__restrict__ partial = J.partial;
+  const int n = J.n;
+  R *__restrict__ proj = J.proj;
+  const R *__restrict__ gm = J.gm;
+  const R *__restrict__ s = J.s;
+  R *__restrict__ pred = J.pred;
+  R *__restrict__ ratio = J.ratio;
+  const R delta = J.delta;
   const int nout = g.jo1 - g.jo0;
   const size_t count = (size_t)n * nout * g.Nz;
-  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (idx >= count) return;
   const int z = idx % g.Nz;
   const int jx = (idx / g.Nz) % nout;
@@ -305,6 +332,21 @@ __global__ __launch_bounds__(256) void fwd_reduce_kernel(Geo<R> g, const R *__re
   }
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void fwd_reduce_kernel(ReduceJob<R> J) {
+  reduce_body<R>(J, (size_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void reduce_batch_kernel(const ReduceJob<R> *__restrict__ jobs,
+                                                           const int *__restrict__ starts,
+                                                           int njobs) {
+  const int b = blockIdx.x;
+  const int j = find_job(starts, njobs, b);
+  const ReduceJob<R> J = ld_uniform_struct(jobs + j);
+  reduce_body<R>(J, (size_t)(b - ld_uniform(starts + j)) * 256 + threadIdx.x);
+}
+
 template <typename R, int H>
 hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *mu,
                         const double *att, bool att_global, const int32_t *idx, int n, R *partial,
@@ -319,8 +361,26 @@ hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const 
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
   }
+  FwdJob<R> J{g, tb, f, mu, att, att_global ? 1 : 0, idx, n, partial};
   dim3 grid(n, g.ntiles, g.C);
-  kern<<<grid, bt, smem, st>>>(g, tb, f, mu, att, att_global ? 1 : 0, idx, partial, n);
+  kern<<<grid, bt, smem, st>>>(J);
+  return hipGetLastError();
+}
+
+template <typename R, int H>
+hipError_t launch_fwd_batch_h(const FwdJob<R> *jobs, const int *starts, int njobs, int blocks,
+                              int Nz, hipStream_t st) {
+  constexpr int RZ = FwdTile<R>::RZ, QS = FwdTile<R>::QS;
+  const int bt = tile_block_threads(Nz, RZ, QS);
+  const int runs = (Nz + RZ - 1) / RZ;
+  const size_t smem = 2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
+  auto kern = fwd_batch_kernel<R, RZ, QS, H>;
+  if (smem > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+  }
+  kern<<<blocks, bt, smem, st>>>(jobs, starts, njobs);
   return hipGetLastError();
 }
 
@@ -360,9 +420,29 @@ hipError_t launch_forward(const Geo<R> &g, const Tables<R> &tb, const R *f, cons
 template <typename R>
 hipError_t launch_forward_reduce(const Geo<R> &g, const R *partial, int n, R *proj, const R *gm,
                                  const R *s, R *pred, R *ratio, R delta, hipStream_t st) {
-  const size_t count = (size_t)n * (g.jo1 - g.jo0) * g.Nz;
-  fwd_reduce_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(g, partial, n, proj, gm, s,
-                                                                         pred, ratio, delta);
+  ReduceJob<R> J{g, partial, n, proj, gm, s, pred, ratio, delta};
+  fwd_reduce_kernel<R><<<reduce_blocks(g, n), 256, 0, st>>>(J);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_forward_batch(const FwdJob<R> *jobs, const int *starts, int njobs, int blocks,
+                                int H, int Nz, hipStream_t st) {
+#define SPECT_FB_CASE(HH) \
+  case HH: return launch_fwd_batch_h<R, HH>(jobs, starts, njobs, blocks, Nz, st);
+  switch (H) {
+    SPECT_FB_CASE(0) SPECT_FB_CASE(2) SPECT_FB_CASE(4) SPECT_FB_CASE(6) SPECT_FB_CASE(8)
+    SPECT_FB_CASE(10) SPECT_FB_CASE(12) SPECT_FB_CASE(14) SPECT_FB_CASE(16) SPECT_FB_CASE(18)
+    SPECT_FB_CASE(20)
+    default: return hipErrorInvalidValue;
+  }
+#undef SPECT_FB_CASE
+}
+
+template <typename R>
+hipError_t launch_reduce_batch(const ReduceJob<R> *jobs, const int *starts, int njobs, int blocks,
+                               hipStream_t st) {
+  reduce_batch_kernel<R><<<blocks, 256, 0, st>>>(jobs, starts, njobs);
   return hipGetLastError();
 }
 
@@ -373,7 +453,11 @@ hipError_t launch_forward_reduce(const Geo<R> &g, const R *partial, int n, R *pr
                                         const R *, const double *, bool, const int32_t *, int, \
                                         R *, hipStream_t);                                     \
   template hipError_t launch_forward_reduce<R>(const Geo<R> &, const R *, int, R *, const R *, \
-                                               const R *, R *, R *, R, hipStream_t);
+                                               const R *, R *, R *, R, hipStream_t);           \
+  template hipError_t launch_forward_batch<R>(const FwdJob<R> *, const int *, int, int, int,    \
+                                              int, hipStream_t);                               \
+  template hipError_t launch_reduce_batch<R>(const ReduceJob<R> *, const int *, int, int,       \
+                                             hipStream_t);
 SPECT_INST(float)
 SPECT_INST(double)
 #undef SPECT_INST

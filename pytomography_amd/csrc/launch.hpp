@@ -10,6 +10,90 @@
 
 namespace spect {
 
+// Work descriptors of one problem for each kernel.  A single-problem launch
+// passes one by value; a batched launch reads an array of them from device
+// memory (one per problem) and maps its blocks onto them.
+template <typename R> struct FwdJob {
+  Geo<R> g;
+  Tables<R> tb;
+  const R *f;
+  const R *mu;
+  const double *att;
+  int att_global;
+  const int32_t *angle_idx;
+  int n;
+  R *partial;
+};
+
+template <typename R> struct ReduceJob {
+  Geo<R> g;
+  const R *partial;
+  int n;
+  R *proj;
+  const R *gm;
+  const R *s;
+  R *pred;
+  R *ratio;
+  R delta;
+};
+
+template <typename R> struct FieldJob {
+  Geo<R> g;
+  Tables<R> tb;
+  const R *proj;
+  const R *mu;
+  const double *att;
+  int att_global;
+  const int32_t *angle_idx;
+  int n;
+  R *W;
+  const R *f;
+  double *Wm;
+  double *Dsum;
+};
+
+template <typename R> struct GatherJob {
+  Geo<R> g;
+  Tables<R> tb;
+  const R *W;
+  const int32_t *angle_idx;
+  int n;
+  int out_mode;
+  R *out;
+  const R *f;
+  const R *nrm;
+  R delta;
+  R floor_value;
+  const double *Wm;
+  const double *Pref;
+  R *out_mu;
+};
+
+// Blocks per job of each kernel (the batched launchers sum these).
+template <typename R> int fwd_blocks(const Geo<R> &g, int n) { return n * g.ntiles * g.C; }
+template <typename R> int reduce_blocks(const Geo<R> &g, int n) {
+  return (int)(((size_t)n * (g.jo1 - g.jo0) * g.Nz + 255) / 256);
+}
+template <typename R> int gather_blocks(const Geo<R> &g) {
+  constexpr int ZV = sizeof(R) == 4 ? 4 : 2;
+  return (int)(((size_t)g.N * g.N * ((g.Nz + ZV - 1) / ZV) + 255) / 256);
+}
+
+// Batched launches: jobs[] and starts[] (first block of each job, ascending)
+// are device arrays; all jobs of one launch share the PSF half width H and Nz.
+template <typename R>
+hipError_t launch_forward_batch(const FwdJob<R> *jobs, const int *starts, int njobs, int blocks,
+                                int H, int Nz, hipStream_t st);
+template <typename R>
+hipError_t launch_reduce_batch(const ReduceJob<R> *jobs, const int *starts, int njobs, int blocks,
+                               hipStream_t st);
+template <typename R>
+hipError_t launch_field_batch(const FieldJob<R> *jobs, const int *starts, int njobs, int blocks,
+                              int H, int Nz, hipStream_t st);
+template <typename R>
+hipError_t launch_gather_batch(const GatherJob<R> *jobs, const int *starts, int njobs, int blocks,
+                               int mode, hipStream_t st);
+
 // Threads per block of the tile kernels for a z extent Nz.
 int tile_block_threads(int Nz, int run_len, int splits);
 
