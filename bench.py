@@ -110,20 +110,200 @@ def cpu_baseline(n: int, n_angles: int, att: bool, psf: bool, seed: int, budget_
                        f"{el:.1f} s, scaled x8 to one iteration")
 
 
+def bench_batch(args, device):
+    """Config 4: 32 problems, N in {64..192}, A in {60, 90, 120}, attenuation +
+    PSF, OSEM 4 x 8 -- batched hipGraph vs the one-at-a-time GPU loop."""
+    import pytomography_amd
+    from pytomography_amd import synthetic
+    from pytomography_amd.algorithms import OSEM
+    from pytomography_amd.batch import BatchedOSEM, autotune_chunk_planes
+    from pytomography_amd.likelihoods import PoissonLogLikelihood
+    from pytomography_amd.metadata.SPECT import SPECTObjectMeta, SPECTProjMeta, SPECTPSFMeta
+    from pytomography_amd.transforms.SPECT import SPECTAttenuationTransform, SPECTPSFTransform
+    from pytomography_amd.projectors.SPECT import SPECTSystemMatrix
+
+    pytomography_amd.set_device(device)
+    rng = np.random.default_rng(2024)
+    specs = [(int(rng.choice([64, 96, 128, 160, 192])), int(rng.choice([60, 90, 120])), s)
+             for s in range(args.batch)]
+    iters, subsets = 4, 8
+
+    def system_matrix(n, na, seed, chunk):
+        mu = torch.from_numpy(synthetic.attenuation_map(n, seed)).to(device)
+        om = SPECTObjectMeta((synthetic.DX,) * 3, (n, n, n))
+        pm = SPECTProjMeta((n, n), (synthetic.DX, synthetic.DX), list(synthetic.angles(na)),
+                           radii=[synthetic.RADIUS] * na)
+        return SPECTSystemMatrix([SPECTAttenuationTransform(mu),
+                                  SPECTPSFTransform(SPECTPSFMeta(synthetic.PSF_PARAMS))], [], om, pm,
+                                 chunk_planes=chunk)
+
+    t0 = time.perf_counter()
+    tuned = {}
+    for n, na, _ in specs:
+        if (n, na) not in tuned:
+            tuned[(n, na)] = autotune_chunk_planes(lambda c, n=n, na=na: system_matrix(n, na, 0, c),
+                                                   n, na, subsets, key=f"att_psf_{n}_{na}_{subsets}",
+                                                   device=device)
+    tune_s = time.perf_counter() - t0
+
+    def likelihoods():
+        out = []
+        for n, na, seed in specs:
+            sm = system_matrix(n, na, seed, tuned[(n, na)])
+            f_true = torch.from_numpy(synthetic.ellipsoid_phantom(n, seed)).to(device)
+            exp = sm.forward(f_true) * 1e3
+            s = torch.full_like(exp, 0.05 * float(exp.mean()))
+            g = torch.poisson(exp + s, generator=torch.Generator(device=device).manual_seed(seed))
+            out.append(PoissonLogLikelihood(sm, g, additive_term=s))
+        return out
+
+    # one-at-a-time GPU loop (same kernels, one problem after another)
+    lks = likelihoods()
+    for lk in lks:
+        lk._set_n_subsets(subsets)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for lk in lks:
+        alg = OSEM(lk)
+        alg._set_n_subsets = lambda k: None
+        for _ in range(iters):
+            for k in range(subsets):
+                alg._fused_subiteration(0, k)
+    torch.cuda.synchronize()
+    single_s = time.perf_counter() - t0
+    del lks
+
+    batch = BatchedOSEM(likelihoods(), subsets, use_graph=True)
+    batch.run(iters)                      # capture + first replay (warm-up)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = args.steps
+    for _ in range(reps):
+        batch.run(iters)
+    torch.cuda.synchronize()
+    batch_s = (time.perf_counter() - t0) / reps
+    n_prob = len(specs)
+    out = {
+        "metric": METRIC,
+        "value": n_prob * iters / batch_s,
+        "unit": "problem OSEM it/s (batched, 32 uneven problems)",
+        "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": 1e3 * batch_s,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantoms, Poisson counts)",
+        "config": {
+            "workload": f"{n_prob} OSEM reconstructions, N in {{64..192}}, A in {{60,90,120}}, "
+                        f"attenuation + PSF, {iters} iterations x {subsets} subsets, one hipGraph",
+            "problems": [[n, na] for n, na, _ in specs],
+            "chunk_planes": {f"{n}x{na}": c for (n, na), c in tuned.items()},
+            "autotune_s": tune_s,
+            "one_at_a_time_s": single_s,
+            "batched_graph_s": batch_s,
+            "speedup_vs_one_at_a_time": single_s / batch_s,
+        },
+    }
+    print(json.dumps(out))
+
+
+def bench_unrolled(args, device):
+    """Config 5: BSREM + RDP (beta 0.1, gamma 2) unrolled for 10 iterations at
+    192^3 x 180 angles with attenuation + PSF; loss ||x_10 - x_true||^2
+    backpropagated to the projections, the initial object and beta, with
+    only the iterates checkpointed.  float32 run timed and compared with the
+    same run in float64."""
+    import pytomography_amd
+    from pytomography_amd.priors import RelativeDifferencePrior
+    from pytomography_amd.unrolled import UnrolledBSREM
+
+    pytomography_amd.set_device(device)
+    n, na, iters, subsets = args.n, args.angles, args.iters, args.subsets
+    prob = build_problem(n, na, True, True, seed=0, device=device)
+    sm = prob["sm"]
+    sm._ws.clear()               # drop the all-angle workspace of the data simulation
+    torch.cuda.empty_cache()
+    f_true = prob["f_true"].to(device)
+    res = {}
+    for dt in (torch.float32, torch.float64):
+        un = UnrolledBSREM(sm, RelativeDifferencePrior(beta=0.1, gamma=2.0),
+                           additive_term=prob["scatter"].to(dt), n_iters=iters, n_subsets=subsets)
+        g = prob["data"].to(dt).detach().clone().requires_grad_(True)
+        x0 = torch.ones(sm._shape, dtype=dt, device=device).requires_grad_(True)
+        beta = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
+        un._norms(dt, device)                      # sensitivities, one-off
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats(device)
+        base = torch.cuda.memory_allocated(device)
+        t0 = time.perf_counter()
+        xk = un(g, x0, beta)
+        loss = ((xk - f_true.to(dt)) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        after_fwd = torch.cuda.memory_allocated(device)
+        peak_fwd = torch.cuda.max_memory_allocated(device)
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        res[dt] = dict(fwd_s=t1 - t0, bwd_s=t2 - t1, loss=float(loss.detach()),
+                       peak=torch.cuda.max_memory_allocated(device), base=base,
+                       after_fwd=after_fwd, peak_fwd=peak_fwd,
+                       g=g.grad.detach(), x0=x0.grad.detach(), beta=float(beta.grad))
+        print(f"[unrolled {dt}] forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s, "
+              f"resident before {base / 2**30:.2f} GiB, after forward {after_fwd / 2**30:.2f} GiB, "
+              f"peak forward {peak_fwd / 2**30:.2f} GiB, peak {res[dt]['peak'] / 2**30:.2f} GiB",
+              file=sys.stderr, flush=True)
+        del un, g, x0, beta, xk, loss
+
+    def rel(a, b):
+        return float((a.double() - b.double()).norm() / b.double().norm())
+    r32, r64 = res[torch.float32], res[torch.float64]
+    step_s = r32["fwd_s"] + r32["bwd_s"]
+    out = {
+        "metric": METRIC,
+        "value": iters / step_s,
+        "unit": "unrolled BSREM iterations/s (forward + reverse)",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * step_s,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantom, water/bone mu, Poisson counts)",
+        "config": {
+            "workload": (f"unrolled BSREM+RDP {n}^3 x {na} angles, attenuation + PSF, "
+                         f"{iters} iterations x {subsets} subsets, loss ||x_K - x_true||^2"),
+            "forward_s": r32["fwd_s"], "reverse_s": r32["bwd_s"],
+            "peak_hbm_gib_f32": r32["peak"] / 2**30,
+            "resident_before_gib_f32": r32["base"] / 2**30,
+            "checkpoints_gib_f32": (r32["after_fwd"] - r32["base"]) / 2**30,
+            "peak_hbm_above_inputs_gib_f32": (r32["peak"] - r32["base"]) / 2**30,
+            "peak_hbm_gib_f64": r64["peak"] / 2**30,
+            "f64_forward_s": r64["fwd_s"], "f64_reverse_s": r64["bwd_s"],
+            "loss_rel_err_f32_vs_f64": abs(r32["loss"] - r64["loss"]) / abs(r64["loss"]),
+            "grad_g_rel_l2_f32_vs_f64": rel(r32["g"], r64["g"]),
+            "grad_x0_rel_l2_f32_vs_f64": rel(r32["x0"], r64["x0"]),
+            "grad_beta_rel_err_f32_vs_f64": abs(r32["beta"] - r64["beta"]) / abs(r64["beta"]),
+            "grad_beta_f64": r64["beta"],
+        },
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["att_psf", "plain"], default="att_psf")
-    ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--angles", type=int, default=120)
-    ap.add_argument("--subsets", type=int, default=8)
+    ap.add_argument("--workload", choices=["att_psf", "plain", "batch", "unrolled"],
+                    default="att_psf")
+    ap.add_argument("--iters", type=int, default=10, help="iterations of the unrolled workload")
+    ap.add_argument("--batch", type=int, default=32, help="problems of the batch workload")
+    ap.add_argument("--n", type=int, default=None, help="object size (128; 192 for unrolled)")
+    ap.add_argument("--angles", type=int, default=None, help="angles (120; 180 for unrolled)")
+    ap.add_argument("--subsets", type=int, default=None, help="subsets (8; 4 for unrolled)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
+    unrolled = args.workload == "unrolled"
+    args.n = args.n or (192 if unrolled else 128)
+    args.angles = args.angles or (180 if unrolled else 120)
+    args.subsets = args.subsets or (4 if unrolled else 8)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -136,6 +316,12 @@ def main():
         sys.exit(2)
     device = torch.device("cuda", 0)
     torch.cuda.set_device(device)
+    if args.workload == "batch":
+        bench_batch(args, device)
+        return
+    if args.workload == "unrolled":
+        bench_unrolled(args, device)
+        return
     att = psf = args.workload == "att_psf"
     n, na, ns = args.n, args.angles, args.subsets
 

@@ -128,6 +128,16 @@ int spect_backward(const spect_plan *p, int dtype, int mode, int out_mode, const
                    void *out, const void *f, const void *norm, double delta, double floor_value,
                    void *workspace, size_t workspace_bytes, void *stream);
 
+/* Transpose of the reference back projection: proj = B_m^T v with
+ * B_m = spect_backward(REFERENCE) (spect_system_matrix.py:159-201), i.e. the
+ * forward projector with the rotation replaced by the transpose of the
+ * gather at 270-beta (spect_system_matrix.py:196).  No counterpart in the
+ * reference; needed to differentiate through OSEM/BSREM iterations.
+ *   v [nx][ny][nz]  ->  proj [n][ny][nz]                               */
+int spect_backward_transpose(const spect_plan *p, int dtype, const void *v, const void *mu,
+                             const void *att_table, const int32_t *angle_idx, int n, void *proj,
+                             void *workspace, size_t workspace_bytes, void *stream);
+
 /* One OSEM sub-iteration in place on f (the body of the subset loop of
  * preconditioned_gradient_ascent.py:103-122 for OSEM with
  * PoissonLogLikelihood.compute_gradient, poisson_log_likelihood.py:32-36):
@@ -200,6 +210,15 @@ int spect_poisson_ratio(int dtype, const void *g, const void *hf, const void *s,
 int spect_rdp_gradient(int dtype, const void *f, void *out, int32_t nx, int32_t ny, int32_t nz,
                        double dx, double dy, double dz, double beta, double gamma,
                        double delta, double scale, void *stream);
+
+/* Transposed Jacobian of spect_rdp_gradient times u (the RDP Hessian-vector
+ * product), built from RelativeDifferencePrior.phi2_1 / phi2_2
+ * (priors/nearest_neighbour.py:179-183) over the replicate-clamped 26
+ * neighbours:  out = beta*scale * J_r^T u.  No counterpart in the reference
+ * (its order-2 prior operator, nearest_neighbour.py:104-107, is not transposed). */
+int spect_rdp_hessian_vjp(int dtype, const void *f, const void *u, void *out, int32_t nx,
+                          int32_t ny, int32_t nz, double dx, double dy, double dz, double beta,
+                          double gamma, double delta, double scale, void *stream);
 
 /* Linear preconditioned update of PreconditionedGradientAscentAlgorithm.__call__
  * (preconditioned_gradient_ascent.py:119-122) for BSREM's preconditioner

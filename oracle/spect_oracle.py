@@ -469,3 +469,28 @@ def bsrem(g: SPECTGeometry, data, n_iters: int, n_subsets: int, beta: float, gam
             f += precond * (grad - prior_grad)
             f[f <= addition_after_iteration] = addition_after_iteration
     return f
+
+
+def bsrem_unrolled(g: SPECTGeometry, data, n_iters: int, n_subsets: int, beta, gamma: float,
+                   additive=None, f0=None, relaxation=lambda n: 1, addition_after_iteration=1e-4):
+    """``bsrem`` written without in-place updates so that torch autograd can
+    differentiate x_K with respect to ``data``, ``f0`` and a tensor ``beta``;
+    the clamp ``f[f <= floor] = floor`` becomes ``where`` (same values)."""
+    f = torch.ones(g.shape, dtype=g.dtype) if f0 is None else f0
+    if additive is None:
+        additive = torch.zeros(data.shape, dtype=g.dtype)
+    subsets = g.subsets(n_subsets) if n_subsets > 1 else [torch.arange(g.n_angles)]
+    with torch.no_grad():
+        norms = [sensitivity(g, s) for s in subsets]
+        norm_sum = torch.stack(norms).sum(axis=0)
+    floor = torch.tensor(addition_after_iteration, dtype=g.dtype)
+    for j in range(n_iters):
+        for k, sidx in enumerate(subsets):
+            wk = len(sidx) / g.n_angles if n_subsets > 1 else 1
+            prior_grad = rdp_gradient(f, beta, gamma, g.dr, wk)
+            pp = forward(g, f, sidx) + additive[sidx]
+            grad = backward(g, data[sidx] / (pp + DELTA), sidx) - norms[k]
+            d = relaxation(j) / (wk * norm_sum + DELTA)
+            f = f + (f * d) * (grad - prior_grad)
+            f = torch.where(f <= floor, floor, f)
+    return f

@@ -22,10 +22,10 @@ OUT_STORE, OUT_ADD, OUT_OSEM = 0, 1, 2
 EXPORTED = (
     "spect_last_error", "spect_version", "spect_plan_create", "spect_plan_destroy",
     "spect_plan_dims", "spect_workspace_bytes", "spect_att_table_bytes", "spect_att_table_build",
-    "spect_forward", "spect_backward", "spect_osem_subiteration",
+    "spect_forward", "spect_backward", "spect_backward_transpose", "spect_osem_subiteration",
     "spect_grad_workspace_bytes", "spect_forward_vjp",
     "spect_batch_create", "spect_batch_run", "spect_batch_destroy",
-    "spect_poisson_ratio", "spect_rdp_gradient", "spect_bsrem_update",
+    "spect_poisson_ratio", "spect_rdp_gradient", "spect_rdp_hessian_vjp", "spect_bsrem_update",
 )
 
 
@@ -92,6 +92,8 @@ def _declare(lib):
     lib.spect_osem_subiteration.argtypes = [vp, ci, vp, vp, vp, vp, ci, vp, vp, vp, vp, dbl, dbl, vp, sz, vp]
     lib.spect_grad_workspace_bytes.restype = sz
     lib.spect_grad_workspace_bytes.argtypes = [vp, ci, ci]
+    lib.spect_backward_transpose.restype = ci
+    lib.spect_backward_transpose.argtypes = [vp, ci, vp, vp, vp, vp, ci, vp, vp, sz, vp]
     lib.spect_forward_vjp.restype = ci
     lib.spect_forward_vjp.argtypes = [vp, ci, vp, vp, vp, vp, ci, vp, vp, vp, sz, vp]
     lib.spect_batch_create.restype = ci
@@ -105,6 +107,9 @@ def _declare(lib):
     lib.spect_poisson_ratio.argtypes = [ci, vp, vp, vp, vp, vp, i64, dbl, vp]
     lib.spect_rdp_gradient.restype = ci
     lib.spect_rdp_gradient.argtypes = [ci, vp, vp, i32, i32, i32, dbl, dbl, dbl, dbl, dbl, dbl, dbl, vp]
+    lib.spect_rdp_hessian_vjp.restype = ci
+    lib.spect_rdp_hessian_vjp.argtypes = [ci, vp, vp, vp, i32, i32, i32, dbl, dbl, dbl, dbl, dbl,
+                                          dbl, dbl, vp]
     lib.spect_bsrem_update.restype = ci
     lib.spect_bsrem_update.argtypes = [ci, vp, vp, vp, vp, vp, i64, dbl, dbl, dbl, dbl, vp]
 

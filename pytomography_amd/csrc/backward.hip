@@ -447,6 +447,7 @@ hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, c
                           const R *f, double *Wm, double *Dsum, hipStream_t st) {
   constexpr int RZ = BpTile<R>::RZ, QS = BpTile<R>::QS;
   const int bt = tile_block_threads(g.Nz, RZ, QS);
+  if (bt > 1024) return hipErrorInvalidConfiguration;  // __launch_bounds__(1024)
   const int runs = (g.Nz + RZ - 1) / RZ;
   const size_t smem = (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
   dim3 grid(n, g.ntiles, g.C);
@@ -518,6 +519,7 @@ hipError_t launch_field_batch_h(const FieldJob<R> *jobs, const int *starts, int 
                                 int Nz, hipStream_t st) {
   constexpr int RZ = BpTile<R>::RZ, QS = BpTile<R>::QS;
   const int bt = tile_block_threads(Nz, RZ, QS);
+  if (bt > 1024) return hipErrorInvalidConfiguration;  // __launch_bounds__(1024)
   const int runs = (Nz + RZ - 1) / RZ;
   const size_t smem = (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
   auto kern = field_batch_kernel<R, RZ, QS, H>;

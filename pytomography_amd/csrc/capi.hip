@@ -205,7 +205,7 @@ int att_rows(const spect_plan *p, const Geo<R> &g, const R *mu, const void *tabl
 template <typename R>
 int forward_impl(const spect_plan *p, const R *f, const R *mu, const void *table,
                  const int32_t *idx, int n, R *proj, const R *gm, const R *s, R *pred, R *ratio,
-                 R delta, unsigned char *ws, hipStream_t st) {
+                 R delta, unsigned char *ws, hipStream_t st, bool trb = false) {
   const Geo<R> g = geo<R>(p);
   const Ws l = ws_layout(p, sizeof(R) == 8 ? SPECT_F64 : SPECT_F32, n);
   const double *att;
@@ -213,7 +213,7 @@ int forward_impl(const spect_plan *p, const R *f, const R *mu, const void *table
   int rc = att_rows<R>(p, g, mu, table, idx, n, ws, l, st, &att, &global);
   if (rc) return rc;
   R *part = reinterpret_cast<R *>(ws + l.part_off);
-  HIP_TRY(launch_forward<R>(g, tables<R>(p), f, mu, att, global, idx, n, part, st));
+  HIP_TRY(launch_forward<R>(g, tables<R>(p), f, mu, att, global, idx, n, part, trb, st));
   HIP_TRY(launch_forward_reduce<R>(g, part, n, proj, gm, s, pred, ratio, delta, st));
   return SPECT_OK;
 }
@@ -647,6 +647,21 @@ int spect_forward(const spect_plan *p, int dtype, const void *f, const void *mu,
                               (double *)proj, nullptr, nullptr, nullptr, nullptr, 0.0, ws, st);
 }
 
+int spect_backward_transpose(const spect_plan *p, int dtype, const void *v, const void *mu,
+                             const void *att_table, const int32_t *angle_idx, int n, void *proj,
+                             void *workspace, size_t workspace_bytes, void *stream) {
+  if (int rc = check_call(p, dtype, n, workspace, workspace_bytes)) return rc;
+  if (!v || !proj || !angle_idx) return fail(SPECT_E_ARG, "null array argument");
+  if (int rc = check_arrays({v, mu, att_table, proj})) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  auto *ws = reinterpret_cast<unsigned char *>(workspace);
+  if (dtype == SPECT_F32)
+    return forward_impl<float>(p, (const float *)v, (const float *)mu, att_table, angle_idx, n,
+                               (float *)proj, nullptr, nullptr, nullptr, nullptr, 0.f, ws, st, true);
+  return forward_impl<double>(p, (const double *)v, (const double *)mu, att_table, angle_idx, n,
+                              (double *)proj, nullptr, nullptr, nullptr, nullptr, 0.0, ws, st, true);
+}
+
 int spect_backward(const spect_plan *p, int dtype, int mode, int out_mode, const void *proj,
                    const void *mu, const void *att_table, const int32_t *angle_idx, int n,
                    void *out, const void *f, const void *norm, double delta, double floor_value,
@@ -754,6 +769,23 @@ int spect_rdp_gradient(int dtype, const void *f, void *out, int32_t nx, int32_t 
   else if (dtype == SPECT_F64)
     HIP_TRY(launch_rdp<double>((const double *)f, (double *)out, nx, ny, nz, dx, dy, dz, bs, gamma,
                                delta, st));
+  else
+    return fail(SPECT_E_ARG, "bad dtype");
+  return SPECT_OK;
+}
+
+int spect_rdp_hessian_vjp(int dtype, const void *f, const void *u, void *out, int32_t nx,
+                          int32_t ny, int32_t nz, double dx, double dy, double dz, double beta,
+                          double gamma, double delta, double scale, void *stream) {
+  if (!f || !u || !out || nx <= 0 || ny <= 0 || nz <= 0) return fail(SPECT_E_ARG, "bad prior arguments");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const double bs = beta * scale;
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_rdp_hvp<float>((const float *)f, (const float *)u, (float *)out, nx, ny, nz, dx,
+                                  dy, dz, (float)bs, (float)gamma, (float)delta, st));
+  else if (dtype == SPECT_F64)
+    HIP_TRY(launch_rdp_hvp<double>((const double *)f, (const double *)u, (double *)out, nx, ny, nz,
+                                   dx, dy, dz, bs, gamma, delta, st));
   else
     return fail(SPECT_E_ARG, "bad dtype");
   return SPECT_OK;

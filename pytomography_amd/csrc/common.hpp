@@ -28,8 +28,10 @@ template <> struct V2<double> { using T = double2; };
 // owns one rotated-frame column and RZ consecutive z values; in the y phase it
 // owns one z value and 1/QS of the tile's rows (back projection) or output
 // columns (forward).
+// Forward blocks are 32 * ceil(Nz / RZ) threads under a 512-thread launch
+// bound: RZ = 16 keeps Nz <= 256 launchable in both precisions.
 template <typename R> struct FwdTile {
-  static constexpr int RZ = sizeof(R) == 4 ? 16 : 8;
+  static constexpr int RZ = 16;
   static constexpr int QS = 2;
 };
 template <typename R> struct BpTile {

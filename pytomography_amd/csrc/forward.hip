@@ -95,13 +95,23 @@ template <int RZ, int H> __device__ __forceinline__ int lds_stride(int runs) {
   return runs * RZ + 2 * H + 1;
 }
 
-template <typename R, int RZ, int QS, int H>
+// TRB: instead of the forward bilinear gather of f, the transpose of the
+// reference back projection's gather at theta_b = -theta_f
+// (spect_system_matrix.py:196): rotated-frame pixel (i, j) collects every
+// object pixel whose theta_b sample has (i, j) among its bilinear taps, with
+// that tap's weight -- the body then computes B^T v (attenuation, PSF, sum).
+constexpr int kTrbMax = 12;  // contributors per rotated-frame pixel (<= 9 in exact arithmetic)
+
+template <typename R, int RZ, int QS, int H, bool TRB>
 __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int c,
                                          unsigned char *smem) {
   constexpr int K = 2 * H + 1;
   constexpr int JO = kTJ + 2 * H;      // output columns touched by the tile
   constexpr int JH = (JO + QS - 1) / QS;  // per split
   __shared__ Src<R> src[kTJ];
+  __shared__ uint32_t trb_off[TRB ? kTJ : 1][kTrbMax];
+  __shared__ R trb_w[TRB ? kTJ : 1][kTrbMax];
+  __shared__ int trb_n[TRB ? kTJ : 1];
   const Geo<R> &g = J.g;
   const Tables<R> &tb = J.tb;
   const R *__restrict__ f = J.f;
@@ -161,6 +171,40 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
       rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], ld_uniform(tb.lin + i), g.half, ix, iy);
       src[tid] = make_src<R>(ix, iy, g.pad);
       inside = in_footprint<R>(ix, iy, g);
+      if constexpr (TRB) {
+        // object pixels (X, Y) near the forward sample position of (i, j)
+        const int j = j0 + tid;
+        const int Xb = int(floor(iy - R(1.5))), Yb = int(floor(ix - R(1.5)));
+        int cnt = 0;
+        for (int dx_ = 0; dx_ < 4; ++dx_) {
+          const int X = Xb + dx_;
+          if ((unsigned)(X - g.pad) >= (unsigned)g.N) continue;
+          for (int dy_ = 0; dy_ < 4; ++dy_) {
+            const int Y = Yb + dy_;
+            if ((unsigned)(Y - g.pad) >= (unsigned)g.N) continue;
+            R jc, ic;
+            rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, jc, ic);
+            const R fjc = floor(jc), fic = floor(ic);
+            const int c0 = int(fjc), r0 = int(fic);
+            const R tx = jc - fjc, ty = ic - fic;
+            R wr, wc;
+            if (r0 == i) wr = R(1) - ty;
+            else if (r0 + 1 == i) wr = ty;
+            else continue;
+            if (c0 == j) wc = R(1) - tx;
+            else if (c0 + 1 == j) wc = tx;
+            else continue;
+            if (cnt < kTrbMax) {
+              trb_off[tid][cnt] = (uint32_t)((X - g.pad) * g.N + (Y - g.pad)) * (uint32_t)Nz;
+              trb_w[tid][cnt] = wr * wc;
+              ++cnt;
+            }
+          }
+        }
+        trb_n[tid] = cnt;
+        // a plane may hold mu taps (forward footprint) without contributors
+        inside = inside || cnt > 0;
+      }
     }
     if (!__syncthreads_or(inside)) continue;
 
@@ -168,7 +212,19 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
     R own[RZ];
     if (mine) {
       const Taps4<R> t = taps4<R>(src[jl], g.N, Nz);
-      gather_run<R, RZ>(f, t, z0, Nz, own);
+      if constexpr (TRB) {
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) own[m] = R(0);
+        const int cnt = trb_n[jl];
+        for (int e = 0; e < cnt; ++e) {
+          const R *p = f + trb_off[jl][e];
+          const R w = trb_w[jl][e];
+#pragma unroll
+          for (int m = 0; m < RZ; ++m) own[m] += (z0 + m < Nz) ? p[z0 + m] * w : R(0);
+        }
+      } else {
+        gather_run<R, RZ>(f, t, z0, Nz, own);
+      }
       if (has_mu) {
         R x[RZ];
         gather_run<R, RZ>(mu, t, z0, Nz, x);
@@ -273,10 +329,10 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   }
 }
 
-template <typename R, int RZ, int QS, int H>
+template <typename R, int RZ, int QS, int H, bool TRB>
 __global__ __launch_bounds__(512) void fwd_kernel(FwdJob<R> J) {
   extern __shared__ __align__(16) unsigned char smem[];
-  fwd_body<R, RZ, QS, H>(J, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+  fwd_body<R, RZ, QS, H, TRB>(J, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 
 // Batched: block b belongs to job find_job(b); within a job the blocks run
@@ -290,7 +346,7 @@ __global__ __launch_bounds__(512) void fwd_batch_kernel(const FwdJob<R> *__restr
   const FwdJob<R> J = ld_uniform_struct(jobs + j);
   const int local = b - ld_uniform(starts + j);
   const int c = local % J.g.C, r = local / J.g.C;
-  fwd_body<R, RZ, QS, H>(J, r / J.g.ntiles, r % J.g.ntiles, c, smem);
+  fwd_body<R, RZ, QS, H, false>(J, r / J.g.ntiles, r % J.g.ntiles, c, smem);
 }
 
 template <typename R>
@@ -350,12 +406,13 @@ __global__ __launch_bounds__(256) void reduce_batch_kernel(const ReduceJob<R> *_
 template <typename R, int H>
 hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *mu,
                         const double *att, bool att_global, const int32_t *idx, int n, R *partial,
-                        hipStream_t st) {
+                        bool trb, hipStream_t st) {
   constexpr int RZ = FwdTile<R>::RZ, QS = FwdTile<R>::QS;
   const int bt = tile_block_threads(g.Nz, RZ, QS);
+  if (bt > 512) return hipErrorInvalidConfiguration;  // __launch_bounds__(512)
   const int runs = (g.Nz + RZ - 1) / RZ;
   const size_t smem = 2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
-  auto kern = fwd_kernel<R, RZ, QS, H>;
+  auto kern = trb ? fwd_kernel<R, RZ, QS, H, true> : fwd_kernel<R, RZ, QS, H, false>;
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
@@ -372,6 +429,7 @@ hipError_t launch_fwd_batch_h(const FwdJob<R> *jobs, const int *starts, int njob
                               int Nz, hipStream_t st) {
   constexpr int RZ = FwdTile<R>::RZ, QS = FwdTile<R>::QS;
   const int bt = tile_block_threads(Nz, RZ, QS);
+  if (bt > 512) return hipErrorInvalidConfiguration;  // __launch_bounds__(512)
   const int runs = (Nz + RZ - 1) / RZ;
   const size_t smem = 2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
   auto kern = fwd_batch_kernel<R, RZ, QS, H>;
@@ -400,21 +458,16 @@ hipError_t launch_att_table(const Geo<R> &g, const Tables<R> &tb, const R *mu,
 template <typename R>
 hipError_t launch_forward(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *mu,
                           const double *att, bool att_global, const int32_t *idx, int n,
-                          R *partial, hipStream_t st) {
+                          R *partial, bool trb, hipStream_t st) {
+#define SPECT_F_CASE(HH) \
+  case HH: return launch_fwd_h<R, HH>(g, tb, f, mu, att, att_global, idx, n, partial, trb, st);
   switch (g.H) {
-    case 0: return launch_fwd_h<R, 0>(g, tb, f, mu, att, att_global, idx, n, partial, st);
-    case 2: return launch_fwd_h<R, 2>(g, tb, f, mu, att, att_global, idx, n, partial, st);
-    case 4: return launch_fwd_h<R, 4>(g, tb, f, mu, att, att_global, idx, n, partial, st);
-    case 6: return launch_fwd_h<R, 6>(g, tb, f, mu, att, att_global, idx, n, partial, st);
-    case 8: return launch_fwd_h<R, 8>(g, tb, f, mu, att, att_global, idx, n, partial, st);
-    case 10: return launch_fwd_h<R, 10>(g, tb, f, mu, att, att_global, idx, n, partial, st);
-    case 12: return launch_fwd_h<R, 12>(g, tb, f, mu, att, att_global, idx, n, partial, st);
-    case 14: return launch_fwd_h<R, 14>(g, tb, f, mu, att, att_global, idx, n, partial, st);
-    case 16: return launch_fwd_h<R, 16>(g, tb, f, mu, att, att_global, idx, n, partial, st);
-    case 18: return launch_fwd_h<R, 18>(g, tb, f, mu, att, att_global, idx, n, partial, st);
-    case 20: return launch_fwd_h<R, 20>(g, tb, f, mu, att, att_global, idx, n, partial, st);
+    SPECT_F_CASE(0) SPECT_F_CASE(2) SPECT_F_CASE(4) SPECT_F_CASE(6) SPECT_F_CASE(8)
+    SPECT_F_CASE(10) SPECT_F_CASE(12) SPECT_F_CASE(14) SPECT_F_CASE(16) SPECT_F_CASE(18)
+    SPECT_F_CASE(20)
     default: return hipErrorInvalidValue;
   }
+#undef SPECT_F_CASE
 }
 
 template <typename R>
@@ -451,7 +504,7 @@ hipError_t launch_reduce_batch(const ReduceJob<R> *jobs, const int *starts, int 
                                           const int32_t *, int, double *, hipStream_t);        \
   template hipError_t launch_forward<R>(const Geo<R> &, const Tables<R> &, const R *,          \
                                         const R *, const double *, bool, const int32_t *, int, \
-                                        R *, hipStream_t);                                     \
+                                        R *, bool, hipStream_t);                               \
   template hipError_t launch_forward_reduce<R>(const Geo<R> &, const R *, int, R *, const R *, \
                                                const R *, R *, R *, R, hipStream_t);           \
   template hipError_t launch_forward_batch<R>(const FwdJob<R> *, const int *, int, int, int,    \

@@ -104,11 +104,13 @@ template <typename R>
 hipError_t launch_att_table(const Geo<R> &g, const Tables<R> &tb, const R *mu,
                             const int32_t *angle_idx, int n, double *table, hipStream_t st);
 
-// Forward projection partials: partial[C][n][ntiles][kTJ + 2H][Nz].
+// Forward projection partials: partial[C][n][ntiles][kTJ + 2H][Nz].  With trb
+// the rotation is the transpose of the reference back projection's gather, so
+// the result is B^T f (B = spect_backward REFERENCE).
 template <typename R>
 hipError_t launch_forward(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *mu,
                           const double *att, bool att_global, const int32_t *angle_idx, int n,
-                          R *partial, hipStream_t st);
+                          R *partial, bool trb, hipStream_t st);
 
 // proj[n][jo1-jo0][Nz] = sum of the partials; if g_meas != nullptr also
 // pred = proj + s (s may be null) and ratio = g_meas / (pred + delta).
@@ -145,6 +147,9 @@ hipError_t launch_ratio(const R *g, const R *hf, const R *s, R *pred, R *ratio, 
 template <typename R>
 hipError_t launch_rdp(const R *f, R *out, int nx, int ny, int nz, double dx, double dy, double dz,
                       R beta_scale, R gamma, R delta, hipStream_t st);
+template <typename R>
+hipError_t launch_rdp_hvp(const R *f, const R *u, R *out, int nx, int ny, int nz, double dx,
+                          double dy, double dz, R c, R gamma, R delta, hipStream_t st);
 template <typename R>
 hipError_t launch_bsrem_update(R *f, const R *b, const R *nrm, const R *nsum, const R *prior,
                                int64_t count, R relax, R weight, R delta, R floor_value,

@@ -53,6 +53,74 @@ __global__ __launch_bounds__(256) void rdp_kernel(const R *__restrict__ f, R *__
   out[idx] = beta_scale * acc;
 }
 
+// Transposed Jacobian of the RDP gradient times u: with r_r = c sum_s w_s
+// phi1(f_r, f_nb(r,s)) (nb = replicate-clamped neighbour),
+//   (J^T u)_q = c sum_s w_s [ u_q phi2_1(f_q, f_nb(q,s))
+//                             + sum_{r: nb(r,s) = q} u_r phi2_2(f_r, f_q) ]
+// with phi2_1 = d phi1 / d f_r and phi2_2 = d phi1 / d f_s as in
+// RelativeDifferencePrior.phi2_1 / phi2_2 (priors/nearest_neighbour.py:179-183).
+// The second sum is a gather: per axis, the r with clamp(r + d) = q.
+template <typename R>
+__global__ __launch_bounds__(256) void rdp_hvp_kernel(const R *__restrict__ f,
+                                                      const R *__restrict__ u,
+                                                      R *__restrict__ out, int nx, int ny, int nz,
+                                                      double dx, double dy, double dz, R c,
+                                                      R gamma, R delta) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t count = (int64_t)nx * ny * nz;
+  if (idx >= count) return;
+  const int z = idx % nz;
+  const int y = (idx / nz) % ny;
+  const int x = idx / ((int64_t)nz * ny);
+  const int n3[3] = {nx, ny, nz};
+  const int q3[3] = {x, y, z};
+  const R fq = f[idx], uq = u[idx];
+  auto phi2_1 = [&](R fr, R fs) {
+    const R den = gamma * fabs(fr - fs) + fr + fs + delta;
+    const R a = R(2) * fs + delta;
+    return (R(2) * a * a) / (den * den * den);
+  };
+  auto phi2_2 = [&](R fr, R fs) {
+    const R den = gamma * fabs(fr - fs) + fr + fs + delta;
+    return -(R(2) * (R(2) * fr + delta) * (R(2) * fs + delta)) / (den * den * den);
+  };
+  R acc = R(0);
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j)
+      for (int k = -1; k <= 1; ++k) {
+        if (i == 0 && j == 0 && k == 0) continue;
+        const double w = dx / sqrt((dx * i) * (dx * i) + (dy * j) * (dy * j) + (dz * k) * (dz * k));
+        const int d3[3] = {i, j, k};
+        // own term: neighbour of q
+        const int xs = min(max(x + i, 0), nx - 1), ys = min(max(y + j, 0), ny - 1),
+                  zs = min(max(z + k, 0), nz - 1);
+        R term = uq * phi2_1(fq, f[((int64_t)xs * ny + ys) * nz + zs]);
+        // voxels r whose clamped neighbour along (i, j, k) is q
+        int cand[3][2], nc[3];
+        for (int a = 0; a < 3; ++a) {
+          nc[a] = 0;
+          const int d = d3[a], qa = q3[a], na = n3[a];
+          if (d == 0) {
+            cand[a][nc[a]++] = qa;
+          } else if (d == 1) {
+            if (qa >= 1) cand[a][nc[a]++] = qa - 1;
+            if (qa == na - 1) cand[a][nc[a]++] = na - 1;
+          } else {
+            if (qa <= na - 2) cand[a][nc[a]++] = qa + 1;
+            if (qa == 0) cand[a][nc[a]++] = 0;
+          }
+        }
+        for (int a = 0; a < nc[0]; ++a)
+          for (int b = 0; b < nc[1]; ++b)
+            for (int e = 0; e < nc[2]; ++e) {
+              const int64_t r = ((int64_t)cand[0][a] * ny + cand[1][b]) * nz + cand[2][e];
+              term += u[r] * phi2_2(f[r], fq);
+            }
+        acc += term * R(w);
+      }
+  out[idx] = c * acc;
+}
+
 // PreconditionedGradientAscentAlgorithm.__call__ (preconditioned_gradient_ascent.py:119-122)
 // with BSREM's preconditioner (:461-474).
 template <typename R>
@@ -95,6 +163,15 @@ hipError_t launch_rdp(const R *f, R *out, int nx, int ny, int nz, double dx, dou
 }
 
 template <typename R>
+hipError_t launch_rdp_hvp(const R *f, const R *u, R *out, int nx, int ny, int nz, double dx,
+                          double dy, double dz, R c, R gamma, R delta, hipStream_t st) {
+  const int64_t count = (int64_t)nx * ny * nz;
+  rdp_hvp_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(f, u, out, nx, ny, nz, dx, dy,
+                                                                     dz, c, gamma, delta);
+  return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_bsrem_update(R *f, const R *b, const R *nrm, const R *nsum, const R *prior,
                                int64_t count, R relax, R weight, R delta, R floor_value,
                                hipStream_t st) {
@@ -108,6 +185,8 @@ hipError_t launch_bsrem_update(R *f, const R *b, const R *nrm, const R *nsum, co
                                       hipStream_t);                                            \
   template hipError_t launch_rdp<R>(const R *, R *, int, int, int, double, double, double, R,  \
                                     R, R, hipStream_t);                                        \
+  template hipError_t launch_rdp_hvp<R>(const R *, const R *, R *, int, int, int, double,      \
+                                        double, double, R, R, R, hipStream_t);                 \
   template hipError_t launch_bsrem_update<R>(R *, const R *, const R *, const R *, const R *,  \
                                              int64_t, R, R, R, R, hipStream_t);
 SPECT_INST(float)

@@ -55,8 +55,10 @@ class RelativeDifferencePrior(NearestNeighbourPrior):
             delta = pytomography_amd.delta
         super().__init__(beta, weight=weight, gamma=gamma, delta=delta)
 
-    def gradient(self, object: torch.Tensor, beta_scale: float) -> torch.Tensor:
-        """beta * beta_scale * sum_s w_rs phi1(f_r, f_s) * FOV_scale."""
+    def gradient(self, object: torch.Tensor, beta_scale: float,
+                 beta: float | None = None) -> torch.Tensor:
+        """beta * beta_scale * sum_s w_rs phi1(f_r, f_s) * FOV_scale;
+        ``beta`` overrides the prior's beta."""
         dt = torch.float64 if object.dtype == torch.float64 else torch.float32
         f = _lib.require_device_tensor(object, "object", dt)
         out = torch.empty_like(f)
@@ -64,11 +66,33 @@ class RelativeDifferencePrior(NearestNeighbourPrior):
         nx, ny, nz = f.shape
         _lib.check(_lib.lib().spect_rdp_gradient(
             _lib.dtype_code(dt), f.data_ptr(), out.data_ptr(), nx, ny, nz, float(m.dx), float(m.dy),
-            float(m.dz), float(self.beta), float(self.gamma), float(self.delta), float(beta_scale),
-            _lib.stream_ptr(f.device)))
+            float(m.dz), float(self.beta if beta is None else beta), float(self.gamma),
+            float(self.delta), float(beta_scale), _lib.stream_ptr(f.device)))
         fov = getattr(self, "FOV_scale", None)
         if fov is not None and not bool(torch.all(fov == 1)):
             out = out * fov
+        return out
+
+    def hessian_vjp(self, object: torch.Tensor, u: torch.Tensor, beta_scale: float,
+                    beta: float | None = None) -> torch.Tensor:
+        """J^T u with J the Jacobian of ``gradient(object, beta_scale)``, from
+        phi2_1 / phi2_2 (nearest_neighbour.py:179-183) over the
+        replicate-clamped neighbours; ``beta`` overrides the prior's beta."""
+        dt = torch.float64 if object.dtype == torch.float64 else torch.float32
+        f = _lib.require_device_tensor(object, "object", dt)
+        uu = _lib.require_device_tensor(u, "u", dt)
+        if uu.shape != f.shape:
+            raise ValueError(f"u shape {tuple(uu.shape)} != object shape {tuple(f.shape)}")
+        fov = getattr(self, "FOV_scale", None)
+        if fov is not None and not bool(torch.all(fov == 1)):
+            uu = (uu * fov).contiguous()
+        out = torch.empty_like(f)
+        m = self.object_meta
+        nx, ny, nz = f.shape
+        _lib.check(_lib.lib().spect_rdp_hessian_vjp(
+            _lib.dtype_code(dt), f.data_ptr(), uu.data_ptr(), out.data_ptr(), nx, ny, nz,
+            float(m.dx), float(m.dy), float(m.dz), float(self.beta if beta is None else beta),
+            float(self.gamma), float(self.delta), float(beta_scale), _lib.stream_ptr(f.device)))
         return out
 
     def value(self, object: torch.Tensor) -> float:

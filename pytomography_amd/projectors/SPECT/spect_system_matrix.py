@@ -249,6 +249,16 @@ class SPECTSystemMatrix(SystemMatrix):
     # ------------------------------------------------------------ projections
     def forward(self, object: torch.Tensor, subset_idx: int | None = None) -> torch.Tensor:
         """H_m f: projections [n, Ly, Lz] (spect_system_matrix.py:120-157)."""
+        return self._project(object, subset_idx, "spect_forward")
+
+    def backward_transpose(self, object: torch.Tensor, subset_idx: int | None = None) -> torch.Tensor:
+        """B_m^T v, the transpose of the reference back projection ``backward``
+        (spect_system_matrix.py:159-201): the forward chain with the rotation
+        replaced by the transpose of the gather at 270-beta.  Used to
+        differentiate through iterations that call ``backward``."""
+        return self._project(object, subset_idx, "spect_backward_transpose")
+
+    def _project(self, object: torch.Tensor, subset_idx, entry: str) -> torch.Tensor:
         device = pytomography_amd.device if object.device.type != "cuda" else object.device
         dt = self._compute_dtype(object)
         f = _lib.require_device_tensor(object.to(device), "object", dt)
@@ -261,10 +271,10 @@ class SPECTSystemMatrix(SystemMatrix):
         tab = self._att_table(device, dt)
         out = torch.empty((n, self._shape[1], self._shape[2]), dtype=dt, device=device)
         ws, nbytes = self._workspace(device, dt, n)
-        _lib.check(plan.lib.spect_forward(plan.handle, _lib.dtype_code(dt), f.data_ptr(),
-                                          _lib.ptr(mu), _lib.ptr(tab), idx.data_ptr(), n,
-                                          out.data_ptr(), ws.data_ptr(), nbytes,
-                                          _lib.stream_ptr(device)))
+        _lib.check(getattr(plan.lib, entry)(plan.handle, _lib.dtype_code(dt), f.data_ptr(),
+                                            _lib.ptr(mu), _lib.ptr(tab), idx.data_ptr(), n,
+                                            out.data_ptr(), ws.data_ptr(), nbytes,
+                                            _lib.stream_ptr(device)))
         return out
 
     def _backward(self, proj, subset_idx, mode, out_mode=_lib.OUT_STORE, out=None, f=None,

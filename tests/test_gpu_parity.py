@@ -101,8 +101,11 @@ def test_adjoint_identity_fp64(cuda, case):
     assert rel <= 1e-6
 
 
-def test_fp64_forward_matches_fp64_oracle(cuda):
-    pa = problem_arrays(24, 12, seed=8, att=True, psf=True)
+@pytest.mark.parametrize("nz", [24, 160])
+def test_fp64_forward_matches_fp64_oracle(cuda, nz):
+    """nz = 160 needs ceil(160 / RZ) z runs per block within the forward
+    kernel's 512-thread launch bound."""
+    pa = problem_arrays(24, 12, seed=8, nz=nz, att=True, psf=True)
     sm = product_system_matrix(pa)
     f = torch.from_numpy(pa["f"]).double()
     out = sm.forward(f.to(cuda)).cpu()

@@ -1,0 +1,156 @@
+"""Differentiating through BSREM + RDP iterations.
+
+* B^T (spect_backward_transpose) against the oracle's torch-autograd VJP of
+  the reference back projection, and the identity <B y, x> = <y, B^T x>;
+* the RDP Hessian-vector product (spect_rdp_hessian_vjp) against autograd
+  through the oracle's prior gradient;
+* the unrolled BSREM gradient (w.r.t. projections, initial object and beta)
+  against the oracle's autograd through its in-place-free BSREM, in float64,
+  and the float32 run against the float64 run.
+"""
+from __future__ import annotations
+
+import pytest
+import torch
+
+from oracle import spect_oracle as so
+from tests.helpers import oracle_geometry, problem_arrays, product_system_matrix, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cuda():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda")
+
+
+CASES = [
+    dict(n=16, n_angles=8, att=False, psf=False),
+    dict(n=24, n_angles=12, att=True, psf=True),
+    dict(n=33, n_angles=10, att=True, psf=True, nz=20),
+]
+
+
+def _ids(c):
+    return f"n{c['n']}a{c['n_angles']}{'_att' if c['att'] else ''}{'_psf' if c['psf'] else ''}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_backward_transpose_matches_oracle_vjp(cuda, case):
+    pa = problem_arrays(case["n"], case["n_angles"], seed=31, nz=case.get("nz"), att=case["att"],
+                        psf=case["psf"])
+    sm = product_system_matrix(pa)
+    gen = torch.Generator().manual_seed(32)
+    v = torch.rand((case["n"], case["n"], pa["nz"]), generator=gen, dtype=torch.float64)
+    geo = oracle_geometry(pa, torch.float64)
+    y = torch.zeros((case["n_angles"], case["n"], pa["nz"]), dtype=torch.float64, requires_grad=True)
+    (ref,) = torch.autograd.grad(so.backward(geo, y), y, v)
+    out64 = sm.backward_transpose(v.to(cuda)).cpu()
+    out32 = sm.backward_transpose(v.float().to(cuda)).cpu()
+    e64, e32 = rel_l2(out64, ref), rel_l2(out32, ref)
+    print(f"[{_ids(case)}] B^T vs oracle autograd: fp64 {e64:.2e}, fp32 {e32:.2e}")
+    assert e64 <= 1e-12 and e32 <= 1e-5
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_backward_transpose_adjoint_identity_fp64(cuda, case):
+    pa = problem_arrays(case["n"], case["n_angles"], seed=33, nz=case.get("nz"), att=case["att"],
+                        psf=case["psf"])
+    sm = product_system_matrix(pa)
+    sm.set_n_subsets(2)
+    gen = torch.Generator().manual_seed(34)
+    for k in (None, 0, 1):
+        n = case["n_angles"] if k is None else len(sm.subset_indices_array[k])
+        x = torch.rand((case["n"], case["n"], pa["nz"]), generator=gen, dtype=torch.float64).to(cuda)
+        y = torch.rand((n, case["n"], pa["nz"]), generator=gen, dtype=torch.float64).to(cuda)
+        lhs = float((sm.backward(y, k) * x).sum())
+        rhs = float((y * sm.backward_transpose(x, k)).sum())
+        rel = abs(lhs - rhs) / abs(lhs)
+        print(f"[{_ids(case)} subset {k}] <By,x>={lhs:.12e} <y,B^T x>={rhs:.12e} rel {rel:.1e}")
+        assert rel <= 1e-12
+
+
+def test_rdp_hessian_vjp_matches_autograd(cuda):
+    from pytomography_amd.metadata.SPECT import SPECTObjectMeta
+    from pytomography_amd.priors import RelativeDifferencePrior
+    gen = torch.Generator().manual_seed(35)
+    f = torch.rand((20, 18, 16), generator=gen, dtype=torch.float64) + 0.05
+    f[3:7, 4:9, 2:5] = 0.0
+    u = torch.randn((20, 18, 16), generator=gen, dtype=torch.float64)
+    dr = (0.48, 0.5, 0.52)
+    prior = RelativeDifferencePrior(beta=0.1, gamma=2.0)
+    prior.set_object_meta(SPECTObjectMeta(dr, f.shape))
+    fr = f.clone().requires_grad_(True)
+    (ref,) = torch.autograd.grad(so.rdp_gradient(fr, 0.1, 2.0, dr, 0.25), fr, u)
+    for dt, tol in ((torch.float64, 1e-12), (torch.float32, 1e-5)):
+        out = prior.hessian_vjp(f.to(cuda, dt), u.to(cuda, dt), beta_scale=0.25).cpu()
+        e = rel_l2(out, ref)
+        print(f"RDP Hessian-vector product {dt}: rel-L2 vs oracle autograd {e:.2e}")
+        assert e <= tol
+
+
+def _bsrem_case(pa, cuda, dt):
+    g64 = oracle_geometry(pa, torch.float64)
+    f_true = torch.from_numpy(pa["f"]).double()
+    expected = so.forward(g64, f_true) * 50
+    s = torch.full_like(expected, 0.05 * float(expected.mean()))
+    data = torch.poisson(expected + s, generator=torch.Generator().manual_seed(36))
+    return g64, f_true, data, s
+
+
+def test_unrolled_bsrem_gradient_matches_oracle_autograd(cuda):
+    from pytomography_amd.priors import RelativeDifferencePrior
+    from pytomography_amd.unrolled import UnrolledBSREM
+    pa = problem_arrays(24, 12, seed=37, att=True, psf=True)
+    g64, f_true, data, s = _bsrem_case(pa, cuda, torch.float64)
+    x0 = torch.ones(g64.shape, dtype=torch.float64) \
+        + 0.1 * torch.rand(g64.shape, generator=torch.Generator().manual_seed(38), dtype=torch.float64)
+    iters, subsets, beta = 2, 3, 0.1
+
+    dv, fv = data.clone().requires_grad_(True), x0.clone().requires_grad_(True)
+    bv = torch.tensor(beta, dtype=torch.float64, requires_grad=True)
+    xr = so.bsrem_unrolled(g64, dv, iters, subsets, bv, 2.0, additive=s, f0=fv)
+    ((xr - f_true) ** 2).sum().backward()
+
+    sm = product_system_matrix(pa)
+    for dt, tol_x, tol_g in ((torch.float64, 1e-12, 1e-9), (torch.float32, 1e-5, 1e-3)):
+        un = UnrolledBSREM(sm, RelativeDifferencePrior(beta=beta, gamma=2.0),
+                           additive_term=s.to(cuda, dt), n_iters=iters, n_subsets=subsets)
+        d = data.to(cuda, dt).detach().clone().requires_grad_(True)
+        f0 = x0.to(cuda, dt).detach().clone().requires_grad_(True)
+        b = torch.tensor(beta, dtype=torch.float64, requires_grad=True)
+        xk = un(d, f0, b)
+        ((xk - f_true.to(cuda, dt)) ** 2).sum().backward()
+        ex = rel_l2(xk.detach(), xr.detach())
+        eg, ef = rel_l2(d.grad, dv.grad), rel_l2(f0.grad, fv.grad)
+        eb = abs(float(b.grad) - float(bv.grad)) / abs(float(bv.grad))
+        print(f"unrolled BSREM {dt}: x_K {ex:.2e}  d/dg {eg:.2e}  d/dx0 {ef:.2e}  d/dbeta {eb:.2e}")
+        assert ex <= tol_x and eg <= tol_g and ef <= tol_g and eb <= tol_g
+
+
+def test_unrolled_bsrem_fp32_gradient_vs_fp64(cuda):
+    from pytomography_amd.priors import RelativeDifferencePrior
+    from pytomography_amd.unrolled import UnrolledBSREM
+    pa = problem_arrays(48, 40, seed=39, att=True, psf=True)
+    sm = product_system_matrix(pa)
+    f_true = torch.from_numpy(pa["f"]).double().to(cuda)
+    with torch.no_grad():
+        expected = sm.forward(f_true) * 50
+    s = torch.full_like(expected, 0.05 * float(expected.mean()))
+    data = torch.poisson(expected + s, generator=torch.Generator(device=cuda).manual_seed(40))
+    grads = {}
+    for dt in (torch.float64, torch.float32):
+        un = UnrolledBSREM(sm, RelativeDifferencePrior(beta=0.1, gamma=2.0), additive_term=s.to(dt),
+                           n_iters=4, n_subsets=4)
+        d = data.to(dt).detach().clone().requires_grad_(True)
+        b = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
+        xk = un(d, None, b)
+        ((xk - f_true.to(dt)) ** 2).sum().backward()
+        grads[dt] = (xk.detach(), d.grad, b.grad)
+    ex = rel_l2(grads[torch.float32][0], grads[torch.float64][0])
+    eg = rel_l2(grads[torch.float32][1], grads[torch.float64][1])
+    eb = abs(float(grads[torch.float32][2]) - float(grads[torch.float64][2])) / abs(float(grads[torch.float64][2]))
+    print(f"unrolled BSREM float32 vs float64: x_K {ex:.2e}, d/dg {eg:.2e}, d/dbeta {eb:.2e}")
+    assert ex <= 1e-4 and eg <= 1e-3 and eb <= 1e-3

@@ -243,3 +243,60 @@ def test_rdp_gradient_is_half_fd_of_value_in_interior():
         fm[idx] -= h
         fd = (so.rdp_value(fp, beta, gamma, (0.48,) * 3) - so.rdp_value(fm, beta, gamma, (0.48,) * 3)) / (2 * h)
         assert abs(float(fd) - 2 * float(grad[idx])) < 1e-6 * abs(float(fd)) + 1e-9
+
+
+def test_rdp_gradient_jacobian_is_symmetric_fd():
+    """The VJP of the prior gradient (the RDP Hessian applied to u) from
+    autograd agrees with central differences of the gradient."""
+    f = _rand((6, 5, 7), seed=11) + 0.2
+    u = _rand((6, 5, 7), seed=12)
+    d = _rand((6, 5, 7), seed=13)
+    fr = f.clone().requires_grad_(True)
+    r = so.rdp_gradient(fr, 0.2, 2.0, (0.48, 0.5, 0.52), 0.5)
+    (jtu,) = torch.autograd.grad(r, fr, u)
+    h = 1e-6
+    jd = (so.rdp_gradient(f + h * d, 0.2, 2.0, (0.48, 0.5, 0.52), 0.5)
+          - so.rdp_gradient(f - h * d, 0.2, 2.0, (0.48, 0.5, 0.52), 0.5)) / (2 * h)
+    # <u, J d> = <J^T u, d>
+    assert abs(float((u * jd).sum()) - float((jtu * d).sum())) < 1e-7 * abs(float((u * jd).sum()))
+
+
+# ------------------------------------------------------- unrolled BSREM
+def _bsrem_problem():
+    from tests.helpers import oracle_geometry, problem_arrays
+    pa = problem_arrays(10, 6, seed=3, att=True, psf=True)
+    g = oracle_geometry(pa, torch.float64)
+    f_true = torch.from_numpy(pa["f"]).double()
+    expected = so.forward(g, f_true) * 50
+    s = torch.full_like(expected, 0.05 * float(expected.mean()))
+    data = torch.poisson(expected + s, generator=torch.Generator().manual_seed(4))
+    return g, f_true, data, s
+
+
+def test_unrolled_bsrem_values_equal_inplace_bsrem():
+    g, f_true, data, s = _bsrem_problem()
+    a = so.bsrem(g, data, 2, 3, 0.1, 2.0, additive=s)
+    b = so.bsrem_unrolled(g, data, 2, 3, 0.1, 2.0, additive=s)
+    assert torch.equal(a, b)
+
+
+def test_unrolled_bsrem_gradient_matches_finite_differences():
+    g, f_true, data, s = _bsrem_problem()
+    x0 = torch.ones(g.shape, dtype=torch.float64) + 0.1 * _rand(g.shape, seed=5)
+
+    def loss(dat, f0, beta):
+        x = so.bsrem_unrolled(g, dat, 2, 3, beta, 2.0, additive=s, f0=f0)
+        return ((x - f_true) ** 2).sum()
+
+    dv, fv = data.clone().requires_grad_(True), x0.clone().requires_grad_(True)
+    bv = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
+    loss(dv, fv, bv).backward()
+    h = 1e-5
+    dd = _rand(data.shape, seed=6)
+    df = _rand(g.shape, seed=7)
+    with torch.no_grad():
+        fd_d = (loss(data + h * dd, x0, 0.1) - loss(data - h * dd, x0, 0.1)) / (2 * h)
+        fd_f = (loss(data, x0 + h * df, 0.1) - loss(data, x0 - h * df, 0.1)) / (2 * h)
+        fd_b = (loss(data, x0, 0.1 + h) - loss(data, x0, 0.1 - h)) / (2 * h)
+    for fd, an in ((fd_d, (dv.grad * dd).sum()), (fd_f, (fv.grad * df).sum()), (fd_b, bv.grad)):
+        assert abs(float(fd) - float(an)) <= 1e-5 * abs(float(fd)), (float(fd), float(an))
