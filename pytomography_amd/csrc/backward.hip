@@ -95,6 +95,17 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
   for (int m = 0; m < (GRAD ? RZ : 1); ++m) SD[m] = 0.0;
   const int rix = ld_uniform(tb.radius_index + ang);
   const int i_lo = c * g.L, i_hi = min(g.P, i_lo + g.L);
+  // the chunk's PSF taps (z then y, per plane) staged in LDS: the blur phases
+  // read them as LDS broadcasts instead of waiting on scalar loads
+  R *Wt = U + (size_t)kTJ * ldu;  // [i_hi - i_lo][2K]
+  if constexpr (H > 0) {
+    const int nt = (i_hi - i_lo) * 2 * K;
+    for (int q = tid; q < nt; q += blockDim.x) {
+      const int l = q / (2 * K), r = q - l * (2 * K);
+      const size_t row = ((size_t)rix * g.P + i_lo + l) * K;
+      Wt[q] = r < K ? tb.taps_z[row + r] : tb.taps_r[row + r - K];
+    }
+  }
   int buf = 0;
   for (int i = i_hi - 1; i >= i_lo; --i) {
     int inside = 0;
@@ -109,13 +120,13 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
     if constexpr (H > 0) {
       // ---- A: y blur of the projection with this plane's taps -> U
       if (aact) {
-        const R *wy = tb.taps_r + ((size_t)rix * g.P + i) * K;
+        const R *wy = Wt + (size_t)(i - i_lo) * 2 * K + K;
         R u[HALF_ROWS];
 #pragma unroll
         for (int r = 0; r < HALF_ROWS; ++r) u[r] = R(0);
 #pragma unroll
         for (int t = 0; t < K; ++t) {
-          const R w = ld_uniform(wy + t);
+          const R w = wy[t];
 #pragma unroll
           for (int r = 0; r < HALF_ROWS; ++r) u[r] += w * pw[r + t];
         }
@@ -125,7 +136,7 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
       __syncthreads();
       // ---- B: z blur of the own run
       if (bact) {
-        const R *wz = tb.taps_z + ((size_t)rix * g.P + i) * K;
+        const R *wz = Wt + (size_t)(i - i_lo) * 2 * K;
         const R *ur = U + (size_t)jl * ldu + z0;  // ur[m] = U(z0 - H + m)
         R win[RZ + 2 * H];
 #pragma unroll
@@ -134,7 +145,7 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
         for (int m = 0; m < RZ; ++m) bz[m] = R(0);
 #pragma unroll
         for (int t = 0; t < K; ++t) {
-          const R w = ld_uniform(wz + t);
+          const R w = wz[t];
 #pragma unroll
           for (int m = 0; m < RZ; ++m) bz[m] += w * win[m + t];
         }
@@ -449,7 +460,7 @@ hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, c
   const int bt = tile_block_threads(g.Nz, RZ, QS);
   if (bt > 1024) return hipErrorInvalidConfiguration;  // __launch_bounds__(1024)
   const int runs = (g.Nz + RZ - 1) / RZ;
-  const size_t smem = (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
+  const size_t smem = ((size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)g.L * 2 * (2 * H + 1)) * sizeof(R);
   dim3 grid(n, g.ntiles, g.C);
   if (Wm) {
     auto kern = bp_field_kernel<R, RZ, QS, H, true>;
@@ -516,12 +527,12 @@ hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
 
 template <typename R, int H>
 hipError_t launch_field_batch_h(const FieldJob<R> *jobs, const int *starts, int njobs, int blocks,
-                                int Nz, hipStream_t st) {
+                                int Nz, int max_L, hipStream_t st) {
   constexpr int RZ = BpTile<R>::RZ, QS = BpTile<R>::QS;
   const int bt = tile_block_threads(Nz, RZ, QS);
   if (bt > 1024) return hipErrorInvalidConfiguration;  // __launch_bounds__(1024)
   const int runs = (Nz + RZ - 1) / RZ;
-  const size_t smem = (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
+  const size_t smem = ((size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)max_L * 2 * (2 * H + 1)) * sizeof(R);
   auto kern = field_batch_kernel<R, RZ, QS, H>;
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
@@ -534,9 +545,9 @@ hipError_t launch_field_batch_h(const FieldJob<R> *jobs, const int *starts, int 
 
 template <typename R>
 hipError_t launch_field_batch(const FieldJob<R> *jobs, const int *starts, int njobs, int blocks,
-                              int H, int Nz, hipStream_t st) {
+                              int H, int Nz, int max_L, hipStream_t st) {
 #define SPECT_FLB_CASE(HH) \
-  case HH: return launch_field_batch_h<R, HH>(jobs, starts, njobs, blocks, Nz, st);
+  case HH: return launch_field_batch_h<R, HH>(jobs, starts, njobs, blocks, Nz, max_L, st);
   switch (H) {
     SPECT_FLB_CASE(0) SPECT_FLB_CASE(2) SPECT_FLB_CASE(4) SPECT_FLB_CASE(6) SPECT_FLB_CASE(8)
     SPECT_FLB_CASE(10) SPECT_FLB_CASE(12) SPECT_FLB_CASE(14) SPECT_FLB_CASE(16)
@@ -563,7 +574,7 @@ hipError_t launch_gather_batch(const GatherJob<R> *jobs, const int *starts, int 
                                          R *, const R *, double *, double *, hipStream_t);     \
   template hipError_t launch_chunk_prefix<R>(const Geo<R> &, double *, int, hipStream_t);      \
   template hipError_t launch_field_batch<R>(const FieldJob<R> *, const int *, int, int, int,    \
-                                            int, hipStream_t);                                 \
+                                            int, int, hipStream_t);                                 \
   template hipError_t launch_gather_batch<R>(const GatherJob<R> *, const int *, int, int, int,  \
                                              hipStream_t);                                     \
   template hipError_t launch_bp_gather<R>(const Geo<R> &, const Tables<R> &, const R *,        \

@@ -275,6 +275,7 @@ struct spect_batch {
   struct Group {
     int H, Nz;
     int nprob;
+    int maxL;  // largest chunk length among the members (LDS tap staging)
     // per subset: device offsets (bytes) into `tables` and block counts
     std::vector<size_t> fwd_off, red_off, fld_off, gat_off, starts_off[4];
     std::vector<int> blocks[4];
@@ -316,6 +317,8 @@ int batch_build(spect_batch *b, const spect_batch_problem *probs, int n, double 
     G.H = keys[k].first;
     G.Nz = keys[k].second;
     G.nprob = (int)members[k].size();
+    G.maxL = 0;
+    for (int i : members[k]) G.maxL = std::max(G.maxL, probs[i].plan->L);
     for (int sub = 0; sub < S; ++sub) {
       std::vector<FwdJob<R>> fj;
       std::vector<ReduceJob<R>> rj;
@@ -378,7 +381,7 @@ int batch_launch(spect_batch *b, int n_iters, hipStream_t st) {
       for (const auto &G : b->groups) {
         HIP_TRY(launch_forward_batch<R>(reinterpret_cast<const FwdJob<R> *>(base + G.fwd_off[sub]),
                                         reinterpret_cast<const int *>(base + G.starts_off[0][sub]),
-                                        G.nprob, G.blocks[0][sub], G.H, G.Nz, st));
+                                        G.nprob, G.blocks[0][sub], G.H, G.Nz, G.maxL, st));
       }
       for (const auto &G : b->groups) {
         HIP_TRY(launch_reduce_batch<R>(reinterpret_cast<const ReduceJob<R> *>(base + G.red_off[sub]),
@@ -388,7 +391,7 @@ int batch_launch(spect_batch *b, int n_iters, hipStream_t st) {
       for (const auto &G : b->groups) {
         HIP_TRY(launch_field_batch<R>(reinterpret_cast<const FieldJob<R> *>(base + G.fld_off[sub]),
                                       reinterpret_cast<const int *>(base + G.starts_off[2][sub]),
-                                      G.nprob, G.blocks[2][sub], G.H, G.Nz, st));
+                                      G.nprob, G.blocks[2][sub], G.H, G.Nz, G.maxL, st));
       }
       for (const auto &G : b->groups) {
         HIP_TRY(launch_gather_batch<R>(reinterpret_cast<const GatherJob<R> *>(base + G.gat_off[sub]),
@@ -495,6 +498,8 @@ int spect_plan_create(const spect_geom *g, spect_plan **out) {
   if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0) return fail(SPECT_E_ARG, "object shape must be positive");
   if (g->nx != g->ny) return fail(SPECT_E_ARG, "nx must equal ny (square transaxial grid)");
   if (g->nz > 256) return fail(SPECT_E_ARG, "nz > 256 is not supported by the tile kernels");
+  if (g->chunk_planes < 0 || g->chunk_planes > 64)
+    return fail(SPECT_E_ARG, "chunk_planes must be 0 (default) or 1..64");
   if (g->n_angles <= 0 || !g->angles_deg) return fail(SPECT_E_ARG, "angles required");
   if (!(g->dx > 0) || !(g->dz > 0)) return fail(SPECT_E_ARG, "voxel sizes must be positive");
   if (g->psf_kr < 0 || g->psf_kz < 0 || (g->psf_kr > 0 && g->psf_kr % 2 == 0) ||

@@ -195,6 +195,56 @@ __device__ __forceinline__ void gather_run(const R *__restrict__ vol, const Taps
   }
 }
 
+// ZV consecutive z of one object pixel column starting at z0 (one 16-byte
+// load when Nz is a multiple of ZV, else element loads within Nz).
+template <typename R, int ZV>
+__device__ __forceinline__ void load_zv(const R *__restrict__ p, int z0, int Nz, bool vec,
+                                        R (&v)[ZV]) {
+  if (vec) {
+    if constexpr (sizeof(R) == 4) {
+      const float4 x = *reinterpret_cast<const float4 *>(p + z0);
+      v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+    } else {
+      const double2 x = *reinterpret_cast<const double2 *>(p + z0);
+      v[0] = x.x; v[1] = x.y;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < ZV; ++e) v[e] = z0 + e < Nz ? p[z0 + e] : R(0);
+  }
+}
+
+// Bilinear sample of ZV consecutive z, in interp()'s order.
+template <typename R, int ZV>
+__device__ __forceinline__ void interp_zv(const R *__restrict__ vol, const Taps4<R> &t, int z0,
+                                          int Nz, bool vec, R (&out)[ZV]) {
+  R v[4][ZV];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) load_zv<R, ZV>(vol + t.o[q], z0, Nz, vec, v[q]);
+#pragma unroll
+  for (int e = 0; e < ZV; ++e) {
+    R a = v[0][e] * t.w[0];
+    a = a + v[1][e] * t.w[1];
+    a = a + v[2][e] * t.w[2];
+    a = a + v[3][e] * t.w[3];
+    out[e] = a;
+  }
+}
+
+// The forward kernel's phase A covers kTJ columns in NPA passes of
+// blockDim / ceil(Nz / ZV) columns; blocks have >= QS * Nz threads, so
+// NPA = kTJ / (QS * ZV) passes suffice (checked at launch).
+template <typename R, int QS>
+__host__ __device__ constexpr int phase_a_passes() {
+  return kTJ / (QS * (16 / (int)sizeof(R))) > 0 ? kTJ / (QS * (16 / (int)sizeof(R))) : 1;
+}
+template <typename R, int QS>
+inline bool phase_a_fits(int block_threads, int Nz) {
+  constexpr int ZV = 16 / sizeof(R);
+  const int nq = (Nz + ZV - 1) / ZV;
+  return (block_threads / nq) * phase_a_passes<R, QS>() >= kTJ;
+}
+
 __device__ __forceinline__ float att_exp(float v) { return __expf(v); }
 __device__ __forceinline__ double att_exp(double v) { return exp(v); }
 

@@ -108,10 +108,13 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   constexpr int K = 2 * H + 1;
   constexpr int JO = kTJ + 2 * H;      // output columns touched by the tile
   constexpr int JH = (JO + QS - 1) / QS;  // per split
-  __shared__ Src<R> src[kTJ];
-  __shared__ uint32_t trb_off[TRB ? kTJ : 1][kTrbMax];
-  __shared__ R trb_w[TRB ? kTJ : 1][kTrbMax];
-  __shared__ int trb_n[TRB ? kTJ : 1];
+  // bilinear taps per tile column, double buffered: the next plane's are
+  // built while the current plane is gathered
+  __shared__ Taps4<R> tp[2][kTJ];
+  __shared__ uint32_t trb_off[TRB ? 2 : 1][TRB ? kTJ : 1][kTrbMax];
+  __shared__ R trb_w[TRB ? 2 : 1][TRB ? kTJ : 1][kTrbMax];
+  __shared__ int trb_n[TRB ? 2 : 1][TRB ? kTJ : 1];
+  __shared__ unsigned long long act_s;  // planes of the chunk with a non-empty footprint
   const Geo<R> &g = J.g;
   const Tables<R> &tb = J.tb;
   const R *__restrict__ f = J.f;
@@ -132,10 +135,20 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   const int j0 = g.je0 + jt * kTJ;
   const int tj = min(kTJ, g.je1 - j0);
   const int n_jext = g.je1 - g.je0;
-  // phase A/B ownership: column jl, z run [z0, z0 + RZ)
+  // phase B ownership: column jl, z run [z0, z0 + RZ)
   const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
   const bool ab = run < runs;
-  const bool mine = ab && jl < tj;
+  // phase A ownership: ZV consecutive z (one 16-byte load) per lane with the
+  // lanes along z, so every wave load covers whole 64-byte sectors; the
+  // thread covers columns aco + p * cpp, p < NPA (cpp * NPA >= kTJ is
+  // checked at launch)
+  constexpr int ZV = 16 / sizeof(R);
+  constexpr int NPA = phase_a_passes<R, QS>();
+  const int nq = (Nz + ZV - 1) / ZV;
+  const int cpp = blockDim.x / nq;
+  const int aco = tid / nq, za = (tid - aco * nq) * ZV;
+  const bool aact = aco < cpp;
+  const bool zvec = (Nz % ZV) == 0;
   // phase C ownership: one z, 1/QS of the output columns (wave-uniform split)
   const int nzc = (Nz + 63) / 64 * 64;
   const int half = tid / nzc, zc = tid - half * nzc;
@@ -143,19 +156,20 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   const auto cs = ld_uniform(tb.trig + ang);
   const bool has_mu = mu != nullptr;
 
-  if constexpr (H > 0) {  // zero the halo pads of V once
-    for (int q = tid; q < kTJ * 2 * H; q += blockDim.x) {
-      const int r = q / (2 * H), m = q - r * 2 * H;
-      V[(size_t)r * ldv + (m < H ? m : runs * RZ + m)] = R(0);
-    }
-  }
-  double S[RZ];
+  // V and Vz start zeroed: halo pads, columns >= tj and z slots >= Nz are
+  // never written afterwards
+  for (int q = tid; q < 2 * kTJ * ldv; q += blockDim.x) V[q] = R(0);
+  double S[NPA][ZV];
 #pragma unroll
-  for (int m = 0; m < RZ; ++m) {
-    S[m] = 0.0;
-    if (has_mu && mine && z0 + m < Nz) {
-      const int rowa = att_global ? ang : al;
-      S[m] = att[(((size_t)rowa * g.C + c) * n_jext + (j0 - g.je0 + jl)) * Nz + z0 + m];
+  for (int p = 0; p < NPA; ++p) {
+    const int jj = aco + p * cpp;
+#pragma unroll
+    for (int e = 0; e < ZV; ++e) {
+      S[p][e] = 0.0;
+      if (has_mu && aact && jj < tj && za + e < Nz) {
+        const int rowa = att_global ? ang : al;
+        S[p][e] = att[(((size_t)rowa * g.C + c) * n_jext + (j0 - g.je0 + jj)) * Nz + za + e];
+      }
     }
   }
   R acc[JH];
@@ -164,86 +178,123 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
 
   const int rix = ld_uniform(tb.radius_index + ang);
   const int i_lo = c * g.L, i_hi = min(g.P, i_lo + g.L);
-  for (int i = i_hi - 1; i >= i_lo; --i) {
-    int inside = 0;
-    if (tid < tj) {
+  // the chunk's PSF taps (z then y, per plane) staged in LDS: the blur phases
+  // read them as LDS broadcasts instead of waiting on scalar loads
+  R *Wt = Vz + (size_t)kTJ * ldv;  // [i_hi - i_lo][2K]
+  if constexpr (H > 0) {
+    const int nt = (i_hi - i_lo) * 2 * K;
+    for (int q = tid; q < nt; q += blockDim.x) {
+      const int l = q / (2 * K), r = q - l * (2 * K);
+      const size_t row = ((size_t)rix * g.P + i_lo + l) * K;
+      Wt[q] = r < K ? tb.taps_z[row + r] : tb.taps_r[row + r - K];
+    }
+  }
+  // which planes of the chunk can touch the object (bit l: plane i_lo + l);
+  // the skipped planes sample outside the object, where f and mu vanish
+  if (tid == 0) act_s = 0ull;
+  __syncthreads();
+  for (int q = tid; q < (i_hi - i_lo) * kTJ; q += blockDim.x) {
+    const int l = q / kTJ, jj = q - l * kTJ;
+    if (jj < tj) {
       R ix, iy;
-      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], ld_uniform(tb.lin + i), g.half, ix, iy);
-      src[tid] = make_src<R>(ix, iy, g.pad);
-      inside = in_footprint<R>(ix, iy, g);
-      if constexpr (TRB) {
-        // object pixels (X, Y) near the forward sample position of (i, j)
-        const int j = j0 + tid;
-        const int Xb = int(floor(iy - R(1.5))), Yb = int(floor(ix - R(1.5)));
-        int cnt = 0;
-        for (int dx_ = 0; dx_ < 4; ++dx_) {
-          const int X = Xb + dx_;
-          if ((unsigned)(X - g.pad) >= (unsigned)g.N) continue;
-          for (int dy_ = 0; dy_ < 4; ++dy_) {
-            const int Y = Yb + dy_;
-            if ((unsigned)(Y - g.pad) >= (unsigned)g.N) continue;
-            R jc, ic;
-            rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, jc, ic);
-            const R fjc = floor(jc), fic = floor(ic);
-            const int c0 = int(fjc), r0 = int(fic);
-            const R tx = jc - fjc, ty = ic - fic;
-            R wr, wc;
-            if (r0 == i) wr = R(1) - ty;
-            else if (r0 + 1 == i) wr = ty;
-            else continue;
-            if (c0 == j) wc = R(1) - tx;
-            else if (c0 + 1 == j) wc = tx;
-            else continue;
-            if (cnt < kTrbMax) {
-              trb_off[tid][cnt] = (uint32_t)((X - g.pad) * g.N + (Y - g.pad)) * (uint32_t)Nz;
-              trb_w[tid][cnt] = wr * wc;
-              ++cnt;
-            }
+      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + jj], tb.lin[i_lo + l], g.half, ix, iy);
+      if (in_footprint<R>(ix, iy, g)) atomicOr(&act_s, 1ull << l);
+    }
+  }
+  const R lin_j = tid < tj ? tb.lin[j0 + tid] : R(0);
+  // taps (and for TRB the contributor lists) of plane i into buffer b; run by tid < tj
+  auto setup = [&](int i, int b) {
+    R ix, iy;
+    rot_coord<R>(cs.x, cs.y, lin_j, ld_uniform(tb.lin + i), g.half, ix, iy);
+    tp[b][tid] = taps4<R>(make_src<R>(ix, iy, g.pad), g.N, Nz);
+    if constexpr (TRB) {
+      // object pixels (X, Y) near the forward sample position of (i, j)
+      const int j = j0 + tid;
+      const int Xb = int(floor(iy - R(1.5))), Yb = int(floor(ix - R(1.5)));
+      int cnt = 0;
+      for (int dx_ = 0; dx_ < 4; ++dx_) {
+        const int X = Xb + dx_;
+        if ((unsigned)(X - g.pad) >= (unsigned)g.N) continue;
+        for (int dy_ = 0; dy_ < 4; ++dy_) {
+          const int Y = Yb + dy_;
+          if ((unsigned)(Y - g.pad) >= (unsigned)g.N) continue;
+          R jc, ic;
+          rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, jc, ic);
+          const R fjc = floor(jc), fic = floor(ic);
+          const int c0 = int(fjc), r0 = int(fic);
+          const R tx = jc - fjc, ty = ic - fic;
+          R wr, wc;
+          if (r0 == i) wr = R(1) - ty;
+          else if (r0 + 1 == i) wr = ty;
+          else continue;
+          if (c0 == j) wc = R(1) - tx;
+          else if (c0 + 1 == j) wc = tx;
+          else continue;
+          if (cnt < kTrbMax) {
+            trb_off[b][tid][cnt] = (uint32_t)((X - g.pad) * g.N + (Y - g.pad)) * (uint32_t)Nz;
+            trb_w[b][tid][cnt] = wr * wc;
+            ++cnt;
           }
         }
-        trb_n[tid] = cnt;
-        // a plane may hold mu taps (forward footprint) without contributors
-        inside = inside || cnt > 0;
+      }
+      // a contributor's forward sample lies within sqrt(2) px of the object,
+      // inside the footprint square, so the plane is in act
+      trb_n[b][tid] = cnt;
+    }
+  };
+  __syncthreads();
+  unsigned long long rem = act_s;
+  int i = rem ? i_lo + 63 - __clzll(rem) : -1;
+  if (i >= 0) {
+    rem &= ~(1ull << (i - i_lo));
+    if (tid < tj) setup(i, 0);
+  }
+  __syncthreads();
+  int b = 0;
+  while (i >= 0) {
+    const int inext = rem ? i_lo + 63 - __clzll(rem) : -1;
+    if (inext >= 0) rem &= ~(1ull << (inext - i_lo));
+    // ---- A: V = a * f_rot, ZV z per lane, NPA columns per thread
+    if (aact) {
+      R *dstA = (H == 0 ? Vz : V + H) + za;
+#pragma unroll
+      for (int p = 0; p < NPA; ++p) {
+        const int jj = aco + p * cpp;
+        if (jj < tj) {
+          const Taps4<R> t = tp[b][jj];
+          R val[ZV];
+#pragma unroll
+          for (int e = 0; e < ZV; ++e) val[e] = R(0);
+          if constexpr (TRB) {
+            const int cnt = trb_n[b][jj];
+            for (int q = 0; q < cnt; ++q) {
+              R v[ZV];
+              load_zv<R, ZV>(f + trb_off[b][jj][q], za, Nz, zvec, v);
+              const R w = trb_w[b][jj][q];
+#pragma unroll
+              for (int e = 0; e < ZV; ++e) val[e] += v[e] * w;
+            }
+          } else {
+            interp_zv<R, ZV>(f, t, za, Nz, zvec, val);
+          }
+          if (has_mu) {
+            R x[ZV];
+            interp_zv<R, ZV>(mu, t, za, Nz, zvec, x);
+#pragma unroll
+            for (int e = 0; e < ZV; ++e) {
+              const R xm = x[e] * g.dx;
+              S[p][e] += (double)xm;
+              val[e] = val[e] * att_factor<R>(S[p][e], xm);
+            }
+          }
+#pragma unroll
+          for (int e = 0; e < ZV; ++e)
+            if (za + e < Nz) dstA[(size_t)jj * ldv + e] = val[e];
+        }
       }
     }
-    if (!__syncthreads_or(inside)) continue;
-
-    // ---- A: V = a * f_rot for the own column / z run
-    R own[RZ];
-    if (mine) {
-      const Taps4<R> t = taps4<R>(src[jl], g.N, Nz);
-      if constexpr (TRB) {
-#pragma unroll
-        for (int m = 0; m < RZ; ++m) own[m] = R(0);
-        const int cnt = trb_n[jl];
-        for (int e = 0; e < cnt; ++e) {
-          const R *p = f + trb_off[jl][e];
-          const R w = trb_w[jl][e];
-#pragma unroll
-          for (int m = 0; m < RZ; ++m) own[m] += (z0 + m < Nz) ? p[z0 + m] * w : R(0);
-        }
-      } else {
-        gather_run<R, RZ>(f, t, z0, Nz, own);
-      }
-      if (has_mu) {
-        R x[RZ];
-        gather_run<R, RZ>(mu, t, z0, Nz, x);
-#pragma unroll
-        for (int m = 0; m < RZ; ++m) {
-          const R xm = x[m] * g.dx;
-          S[m] += (double)xm;
-          own[m] = own[m] * att_factor<R>(S[m], xm);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int m = 0; m < RZ; ++m) own[m] = R(0);
-    }
+    if (inext >= 0 && tid < tj) setup(inext, b ^ 1);
     if constexpr (H == 0) {
-      if (ab) {
-#pragma unroll
-        for (int m = 0; m < RZ; ++m) Vz[(size_t)jl * ldv + z0 + m] = own[m];
-      }
       __syncthreads();
       if (cact) {
         // no PSF: the ray sum of column jl is the plain sum over planes
@@ -255,29 +306,20 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
       }
       __syncthreads();
     } else {
-      if (ab) {
-#pragma unroll
-        for (int m = 0; m < RZ; ++m) V[(size_t)jl * ldv + H + z0 + m] = own[m];
-      }
       __syncthreads();
       // ---- B: z blur (correlation, zero padded) of the own run
-      const R *wz = tb.taps_z + ((size_t)rix * g.P + i) * K;
+      const R *wz = Wt + (size_t)(i - i_lo) * 2 * K;
       if (ab) {
         R win[RZ + 2 * H];
         const R *vr = V + (size_t)jl * ldv + z0;  // win[m] = V(z0 - H + m)
 #pragma unroll
-        for (int m = 0; m < H; ++m) {
-          win[m] = vr[m];
-          win[H + RZ + m] = vr[H + RZ + m];
-        }
-#pragma unroll
-        for (int m = 0; m < RZ; ++m) win[H + m] = own[m];
+        for (int m = 0; m < RZ + 2 * H; ++m) win[m] = vr[m];
         R out[RZ];
 #pragma unroll
         for (int m = 0; m < RZ; ++m) out[m] = R(0);
 #pragma unroll
         for (int t = 0; t < K; ++t) {
-          const R w = ld_uniform(wz + t);
+          const R w = wz[t];
 #pragma unroll
           for (int m = 0; m < RZ; ++m) out[m] += w * win[m + t];
         }
@@ -287,7 +329,7 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
       __syncthreads();
       // ---- C: y blur as a scatter into the tile's output columns
       if (cact) {
-        const R *wy = tb.taps_r + ((size_t)rix * g.P + i) * K;
+        const R *wy = Wt + (size_t)(i - i_lo) * 2 * K + K;
         auto scatter = [&](auto half_c) {
           constexpr int HC = decltype(half_c)::value;
           constexpr int jlo = HC * JH;
@@ -300,7 +342,7 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
           // out[jo] += w[t] * in[jo + t - 2H] (tile-local); t ascending is jv ascending
 #pragma unroll
           for (int t = 0; t < K; ++t) {
-            const R w = ld_uniform(wy + t);
+            const R w = wy[t];
 #pragma unroll
             for (int q = 0; q < JH; ++q) {
               const int jo = jlo + q;
@@ -318,6 +360,8 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
         }
       }
     }
+    i = inext;
+    b ^= 1;
   }
   if (cact) {
     R *dst = partial + ((((size_t)c * n_call + al) * g.ntiles + jt) * JO) * Nz;
@@ -410,8 +454,9 @@ hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const 
   constexpr int RZ = FwdTile<R>::RZ, QS = FwdTile<R>::QS;
   const int bt = tile_block_threads(g.Nz, RZ, QS);
   if (bt > 512) return hipErrorInvalidConfiguration;  // __launch_bounds__(512)
+  if (!phase_a_fits<R, QS>(bt, g.Nz)) return hipErrorInvalidConfiguration;
   const int runs = (g.Nz + RZ - 1) / RZ;
-  const size_t smem = 2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
+  const size_t smem = (2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)g.L * 2 * (2 * H + 1)) * sizeof(R);
   auto kern = trb ? fwd_kernel<R, RZ, QS, H, true> : fwd_kernel<R, RZ, QS, H, false>;
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
@@ -426,12 +471,13 @@ hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const 
 
 template <typename R, int H>
 hipError_t launch_fwd_batch_h(const FwdJob<R> *jobs, const int *starts, int njobs, int blocks,
-                              int Nz, hipStream_t st) {
+                              int Nz, int max_L, hipStream_t st) {
   constexpr int RZ = FwdTile<R>::RZ, QS = FwdTile<R>::QS;
   const int bt = tile_block_threads(Nz, RZ, QS);
   if (bt > 512) return hipErrorInvalidConfiguration;  // __launch_bounds__(512)
+  if (!phase_a_fits<R, QS>(bt, Nz)) return hipErrorInvalidConfiguration;
   const int runs = (Nz + RZ - 1) / RZ;
-  const size_t smem = 2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) * sizeof(R);
+  const size_t smem = (2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)max_L * 2 * (2 * H + 1)) * sizeof(R);
   auto kern = fwd_batch_kernel<R, RZ, QS, H>;
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
@@ -480,9 +526,9 @@ hipError_t launch_forward_reduce(const Geo<R> &g, const R *partial, int n, R *pr
 
 template <typename R>
 hipError_t launch_forward_batch(const FwdJob<R> *jobs, const int *starts, int njobs, int blocks,
-                                int H, int Nz, hipStream_t st) {
+                                int H, int Nz, int max_L, hipStream_t st) {
 #define SPECT_FB_CASE(HH) \
-  case HH: return launch_fwd_batch_h<R, HH>(jobs, starts, njobs, blocks, Nz, st);
+  case HH: return launch_fwd_batch_h<R, HH>(jobs, starts, njobs, blocks, Nz, max_L, st);
   switch (H) {
     SPECT_FB_CASE(0) SPECT_FB_CASE(2) SPECT_FB_CASE(4) SPECT_FB_CASE(6) SPECT_FB_CASE(8)
     SPECT_FB_CASE(10) SPECT_FB_CASE(12) SPECT_FB_CASE(14) SPECT_FB_CASE(16) SPECT_FB_CASE(18)
@@ -508,7 +554,7 @@ hipError_t launch_reduce_batch(const ReduceJob<R> *jobs, const int *starts, int 
   template hipError_t launch_forward_reduce<R>(const Geo<R> &, const R *, int, R *, const R *, \
                                                const R *, R *, R *, R, hipStream_t);           \
   template hipError_t launch_forward_batch<R>(const FwdJob<R> *, const int *, int, int, int,    \
-                                              int, hipStream_t);                               \
+                                              int, int, hipStream_t);                               \
   template hipError_t launch_reduce_batch<R>(const ReduceJob<R> *, const int *, int, int,       \
                                              hipStream_t);
 SPECT_INST(float)

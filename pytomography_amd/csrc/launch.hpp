@@ -83,13 +83,13 @@ template <typename R> int gather_blocks(const Geo<R> &g) {
 // are device arrays; all jobs of one launch share the PSF half width H and Nz.
 template <typename R>
 hipError_t launch_forward_batch(const FwdJob<R> *jobs, const int *starts, int njobs, int blocks,
-                                int H, int Nz, hipStream_t st);
+                                int H, int Nz, int max_L, hipStream_t st);
 template <typename R>
 hipError_t launch_reduce_batch(const ReduceJob<R> *jobs, const int *starts, int njobs, int blocks,
                                hipStream_t st);
 template <typename R>
 hipError_t launch_field_batch(const FieldJob<R> *jobs, const int *starts, int njobs, int blocks,
-                              int H, int Nz, hipStream_t st);
+                              int H, int Nz, int max_L, hipStream_t st);
 template <typename R>
 hipError_t launch_gather_batch(const GatherJob<R> *jobs, const int *starts, int njobs, int blocks,
                                int mode, hipStream_t st);
