@@ -41,6 +41,12 @@ enum {
 
 enum { SPECT_F32 = 0, SPECT_F64 = 1 };
 
+/* Flag OR-ed into the dtype argument of spect_forward, spect_backward,
+ * spect_backward_transpose and spect_osem_subiteration: att_table then holds
+ * the attenuation field of spect_att_field_build instead of the suffix table
+ * (mu must still be non-NULL; it is not read). */
+enum { SPECT_ATT_FIELD = 0x100 };
+
 /* Rotation used by the back projection. */
 enum {
   SPECT_BP_REFERENCE = 0, /* gather at 270-beta: spect_system_matrix.py:196 */
@@ -99,6 +105,17 @@ size_t spect_workspace_bytes(const spect_plan *p, int dtype, int n);
  * rows of its own angles in the workspace. */
 size_t spect_att_table_bytes(const spect_plan *p);
 int spect_att_table_build(const spect_plan *p, int dtype, const void *mu, void *table,
+                          void *stream);
+
+/* Attenuation field: the factor exp(-(S - mu_rot*dx/2)) itself (S = FP64
+ * reverse cumsum, as above; attenuation.py:21 with utils/misc.py:18) at every
+ * rotated-frame voxel the projectors visit, for every angle, in dtype:
+ * [n_angles][P][ny + 2*psf_half][nz] (P = padded width).  With SPECT_ATT_FIELD
+ * the forward and back projections read it instead of resampling mu (same
+ * factors, bitwise); it trades HBM capacity (n_angles * P * (ny + 2*psf_half) *
+ * nz elements) for the per-call gathers of mu. */
+size_t spect_att_field_bytes(const spect_plan *p, int dtype);
+int spect_att_field_build(const spect_plan *p, int dtype, const void *mu, void *field,
                           void *stream);
 
 /* Forward projection H_m f.
@@ -178,6 +195,7 @@ typedef struct spect_batch_problem {
   void *f;                          /* [nx][ny][nz], updated in place */
   const void *mu;                   /* attenuation map or NULL */
   const void *att_table;            /* spect_att_table_build(mu) result; required with mu */
+  int32_t att_field;                /* != 0: att_table is a spect_att_field_build(mu) result */
   int32_t n_subsets;                /* equal for all problems of a batch */
   const int32_t *const *angle_idx;  /* host [n_subsets] of device int32 [count] */
   const int32_t *counts;            /* host [n_subsets] */

@@ -16,12 +16,14 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libspect_hip.so")
 
 SPECT_F32, SPECT_F64 = 0, 1
+ATT_FIELD = 0x100  # OR-ed into dtype: att_table is an attenuation field
 BP_REFERENCE, BP_ADJOINT = 0, 1
 OUT_STORE, OUT_ADD, OUT_OSEM = 0, 1, 2
 
 EXPORTED = (
     "spect_last_error", "spect_version", "spect_plan_create", "spect_plan_destroy",
     "spect_plan_dims", "spect_workspace_bytes", "spect_att_table_bytes", "spect_att_table_build",
+    "spect_att_field_bytes", "spect_att_field_build",
     "spect_forward", "spect_backward", "spect_backward_transpose", "spect_osem_subiteration",
     "spect_grad_workspace_bytes", "spect_forward_vjp",
     "spect_batch_create", "spect_batch_run", "spect_batch_destroy",
@@ -54,7 +56,7 @@ class SpectGeom(ctypes.Structure):
 class SpectBatchProblem(ctypes.Structure):
     _fields_ = [
         ("plan", ctypes.c_void_p), ("f", ctypes.c_void_p), ("mu", ctypes.c_void_p),
-        ("att_table", ctypes.c_void_p), ("n_subsets", ctypes.c_int32),
+        ("att_table", ctypes.c_void_p), ("att_field", ctypes.c_int32), ("n_subsets", ctypes.c_int32),
         ("angle_idx", ctypes.POINTER(ctypes.c_void_p)), ("counts", ctypes.POINTER(ctypes.c_int32)),
         ("g", ctypes.POINTER(ctypes.c_void_p)), ("s", ctypes.POINTER(ctypes.c_void_p)),
         ("norm", ctypes.POINTER(ctypes.c_void_p)), ("pred", ctypes.POINTER(ctypes.c_void_p)),
@@ -84,6 +86,10 @@ def _declare(lib):
     lib.spect_att_table_bytes.argtypes = [vp]
     lib.spect_att_table_build.restype = ci
     lib.spect_att_table_build.argtypes = [vp, ci, vp, vp, vp]
+    lib.spect_att_field_bytes.restype = sz
+    lib.spect_att_field_bytes.argtypes = [vp, ci]
+    lib.spect_att_field_build.restype = ci
+    lib.spect_att_field_build.argtypes = [vp, ci, vp, vp, vp]
     lib.spect_forward.restype = ci
     lib.spect_forward.argtypes = [vp, ci, vp, vp, vp, vp, ci, vp, vp, sz, vp]
     lib.spect_backward.restype = ci

@@ -73,6 +73,7 @@ class BatchedOSEM:
             self._keep.append((idx, counts, g, s, norm, pred, mu, tab, ws, arrs, plan))
             probs[i] = _lib.SpectBatchProblem(
                 plan=plan.handle, f=f.data_ptr(), mu=_lib.ptr(mu), att_table=_lib.ptr(tab),
+                att_field=1 if tab is not None and tab.att_field else 0,
                 n_subsets=n_subsets, angle_idx=arrs["idx"], counts=counts, g=arrs["g"],
                 s=arrs["s"], norm=arrs["norm"], pred=arrs["pred"], workspace=ws.data_ptr(),
                 workspace_bytes=nbytes)

@@ -31,7 +31,7 @@ namespace {
 // sum_{i>k} D(i) + D(k)/2 - sum_i D(i)  (M = mu_rot*dx).  The block writes the
 // chunk-local part Wm(k) = sum_{i>k, i in chunk} D(i) + D(k)/2 (FP64) and the
 // chunk total Dsum (FP64); the gather subtracts the prefix of chunk totals.
-template <typename R, int RZ, int QS, int H, bool GRAD>
+template <typename R, int RZ, int QS, int H, bool GRAD, bool AF>
 __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt, int c,
                                            unsigned char *smem) {
   const Geo<R> &g = J.g;
@@ -40,6 +40,7 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
   const R *__restrict__ mu = J.mu;
   const double *__restrict__ att = J.att;
   const int att_global = J.att_global;
+  const R *__restrict__ attf = J.attf;  // AF: precomputed attenuation factors
   const int32_t *__restrict__ angle_idx = J.angle_idx;
   R *__restrict__ W = J.W;
   const R *__restrict__ f = J.f;
@@ -81,11 +82,11 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
       if (j >= g.jo0 && j < g.jo1) pw[m] = proj[((size_t)al * nout + (j - g.jo0)) * Nz + za];
     }
   }
-  double S[RZ];
+  double S[AF ? 1 : RZ];
 #pragma unroll
-  for (int m = 0; m < RZ; ++m) {
+  for (int m = 0; m < (AF ? 1 : RZ); ++m) {
     S[m] = 0.0;
-    if (has_mu && mine && z0 + m < Nz) {
+    if (!AF && has_mu && mine && z0 + m < Nz) {
       const int rowa = att_global ? ang : al;
       S[m] = att[(((size_t)rowa * g.C + c) * n_jext + (j0 - g.je0 + jl)) * Nz + z0 + m];
     }
@@ -165,7 +166,13 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
     // ---- attenuation and store
     if (mine) {
       Taps4<R> t = taps4<R>(src[buf][jl], g.N, Nz);
-      if (has_mu) {
+      if constexpr (AF) {
+        R a[RZ];
+        load_run<R, RZ>(attf + (((size_t)ang * g.P + i) * n_jext + (j0 - g.je0 + jl)) * Nz, z0, Nz,
+                        a);
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) bz[m] = bz[m] * a[m];
+      } else if (has_mu) {
         R x[RZ];
         gather_run<R, RZ>(mu, t, z0, Nz, x);
 #pragma unroll
@@ -215,13 +222,13 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
   }
 }
 
-template <typename R, int RZ, int QS, int H, bool GRAD>
+template <typename R, int RZ, int QS, int H, bool GRAD, bool AF>
 __global__ __launch_bounds__(1024) void bp_field_kernel(FieldJob<R> J) {
   extern __shared__ __align__(16) unsigned char smem[];
-  field_body<R, RZ, QS, H, GRAD>(J, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+  field_body<R, RZ, QS, H, GRAD, AF>(J, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 
-template <typename R, int RZ, int QS, int H>
+template <typename R, int RZ, int QS, int H, bool AF>
 __global__ __launch_bounds__(1024) void field_batch_kernel(const FieldJob<R> *__restrict__ jobs,
                                                            const int *__restrict__ starts,
                                                            int njobs) {
@@ -231,7 +238,7 @@ __global__ __launch_bounds__(1024) void field_batch_kernel(const FieldJob<R> *__
   const FieldJob<R> J = ld_uniform_struct(jobs + j);
   const int local = b - ld_uniform(starts + j);
   const int c = local % J.g.C, r = local / J.g.C;
-  field_body<R, RZ, QS, H, false>(J, r / J.g.ntiles, r % J.g.ntiles, c, smem);
+  field_body<R, RZ, QS, H, false, AF>(J, r / J.g.ntiles, r % J.g.ntiles, c, smem);
 }
 
 // In place: Dsum[a][c] <- sum_{c' <= c} Dsum[a][c'] (prefix over chunks, FP64).
@@ -454,8 +461,8 @@ __global__ __launch_bounds__(256) void gather_batch_kernel(const GatherJob<R> *_
 
 template <typename R, int H>
 hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
-                          const double *att, bool att_global, const int32_t *idx, int n, R *W,
-                          const R *f, double *Wm, double *Dsum, hipStream_t st) {
+                          const double *att, bool att_global, const R *attf, const int32_t *idx,
+                          int n, R *W, const R *f, double *Wm, double *Dsum, hipStream_t st) {
   constexpr int RZ = BpTile<R>::RZ, QS = BpTile<R>::QS;
   const int bt = tile_block_threads(g.Nz, RZ, QS);
   if (bt > 1024) return hipErrorInvalidConfiguration;  // __launch_bounds__(1024)
@@ -463,22 +470,24 @@ hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, c
   const size_t smem = ((size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)g.L * 2 * (2 * H + 1)) * sizeof(R);
   dim3 grid(n, g.ntiles, g.C);
   if (Wm) {
-    auto kern = bp_field_kernel<R, RZ, QS, H, true>;
-    if (smem > 64 * 1024) {
-      hipError_t e = hipFuncSetAttribute((const void *)kern,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-      if (e != hipSuccess) return e;
-    }
-    kern<<<grid, bt, smem, st>>>(FieldJob<R>{g, tb, proj, mu, att, att_global ? 1 : 0, idx, n, W, f, Wm, Dsum});
-  } else {
-    auto kern = bp_field_kernel<R, RZ, QS, H, false>;
+    if (attf) return hipErrorInvalidValue;  // the gradient field gathers mu itself
+    auto kern = bp_field_kernel<R, RZ, QS, H, true, false>;
     if (smem > 64 * 1024) {
       hipError_t e = hipFuncSetAttribute((const void *)kern,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
       if (e != hipSuccess) return e;
     }
     kern<<<grid, bt, smem, st>>>(
-        FieldJob<R>{g, tb, proj, mu, att, att_global ? 1 : 0, idx, n, W, nullptr, nullptr, nullptr});
+        FieldJob<R>{g, tb, proj, mu, att, att_global ? 1 : 0, nullptr, idx, n, W, f, Wm, Dsum});
+  } else {
+    auto kern = attf ? bp_field_kernel<R, RZ, QS, H, false, true> : bp_field_kernel<R, RZ, QS, H, false, false>;
+    if (smem > 64 * 1024) {
+      hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return e;
+    }
+    kern<<<grid, bt, smem, st>>>(
+        FieldJob<R>{g, tb, proj, mu, att, att_global ? 1 : 0, attf, idx, n, W, nullptr, nullptr, nullptr});
   }
   return hipGetLastError();
 }
@@ -487,10 +496,11 @@ hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, c
 
 template <typename R>
 hipError_t launch_bp_field(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
-                           const double *att, bool att_global, const int32_t *idx, int n, R *W,
-                           const R *f, double *Wm, double *Dsum, hipStream_t st) {
+                           const double *att, bool att_global, const R *attf, const int32_t *idx,
+                           int n, R *W, const R *f, double *Wm, double *Dsum, hipStream_t st) {
 #define SPECT_FIELD_CASE(HH) \
-  case HH: return launch_field_h<R, HH>(g, tb, proj, mu, att, att_global, idx, n, W, f, Wm, Dsum, st);
+  case HH:                   \
+    return launch_field_h<R, HH>(g, tb, proj, mu, att, att_global, attf, idx, n, W, f, Wm, Dsum, st);
   switch (g.H) {
     SPECT_FIELD_CASE(0) SPECT_FIELD_CASE(2) SPECT_FIELD_CASE(4) SPECT_FIELD_CASE(6)
     SPECT_FIELD_CASE(8) SPECT_FIELD_CASE(10) SPECT_FIELD_CASE(12) SPECT_FIELD_CASE(14)
@@ -527,13 +537,13 @@ hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
 
 template <typename R, int H>
 hipError_t launch_field_batch_h(const FieldJob<R> *jobs, const int *starts, int njobs, int blocks,
-                                int Nz, int max_L, hipStream_t st) {
+                                int Nz, int max_L, bool af, hipStream_t st) {
   constexpr int RZ = BpTile<R>::RZ, QS = BpTile<R>::QS;
   const int bt = tile_block_threads(Nz, RZ, QS);
   if (bt > 1024) return hipErrorInvalidConfiguration;  // __launch_bounds__(1024)
   const int runs = (Nz + RZ - 1) / RZ;
   const size_t smem = ((size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)max_L * 2 * (2 * H + 1)) * sizeof(R);
-  auto kern = field_batch_kernel<R, RZ, QS, H>;
+  auto kern = af ? field_batch_kernel<R, RZ, QS, H, true> : field_batch_kernel<R, RZ, QS, H, false>;
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
@@ -545,9 +555,9 @@ hipError_t launch_field_batch_h(const FieldJob<R> *jobs, const int *starts, int 
 
 template <typename R>
 hipError_t launch_field_batch(const FieldJob<R> *jobs, const int *starts, int njobs, int blocks,
-                              int H, int Nz, int max_L, hipStream_t st) {
+                              int H, int Nz, int max_L, bool af, hipStream_t st) {
 #define SPECT_FLB_CASE(HH) \
-  case HH: return launch_field_batch_h<R, HH>(jobs, starts, njobs, blocks, Nz, max_L, st);
+  case HH: return launch_field_batch_h<R, HH>(jobs, starts, njobs, blocks, Nz, max_L, af, st);
   switch (H) {
     SPECT_FLB_CASE(0) SPECT_FLB_CASE(2) SPECT_FLB_CASE(4) SPECT_FLB_CASE(6) SPECT_FLB_CASE(8)
     SPECT_FLB_CASE(10) SPECT_FLB_CASE(12) SPECT_FLB_CASE(14) SPECT_FLB_CASE(16)
@@ -570,11 +580,12 @@ hipError_t launch_gather_batch(const GatherJob<R> *jobs, const int *starts, int 
 
 #define SPECT_INST(R)                                                                          \
   template hipError_t launch_bp_field<R>(const Geo<R> &, const Tables<R> &, const R *,         \
-                                         const R *, const double *, bool, const int32_t *, int, \
-                                         R *, const R *, double *, double *, hipStream_t);     \
+                                         const R *, const double *, bool, const R *,            \
+                                         const int32_t *, int, R *, const R *, double *,        \
+                                         double *, hipStream_t);                                \
   template hipError_t launch_chunk_prefix<R>(const Geo<R> &, double *, int, hipStream_t);      \
   template hipError_t launch_field_batch<R>(const FieldJob<R> *, const int *, int, int, int,    \
-                                            int, int, hipStream_t);                                 \
+                                            int, int, bool, hipStream_t);                      \
   template hipError_t launch_gather_batch<R>(const GatherJob<R> *, const int *, int, int, int,  \
                                              hipStream_t);                                     \
   template hipError_t launch_bp_gather<R>(const Geo<R> &, const Tables<R> &, const R *,        \

@@ -185,11 +185,17 @@ int check_arrays(std::initializer_list<const void *> arrays) {
 // Attenuation rows for the call: the caller's table (indexed by global angle)
 // or rows built now in the workspace (indexed by position in angle_idx).
 template <typename R>
-int att_rows(const spect_plan *p, const Geo<R> &g, const R *mu, const void *table,
+int att_rows(const spect_plan *p, const Geo<R> &g, const R *mu, const void *table, bool field,
              const int32_t *idx, int n, unsigned char *ws, const Ws &l, hipStream_t st,
-             const double **att, bool *global) {
+             const double **att, bool *global, const R **attf) {
   *att = nullptr;
   *global = false;
+  *attf = nullptr;
+  if (field) {  // precomputed attenuation factors of every angle
+    if (!table) return fail(SPECT_E_ARG, "SPECT_ATT_FIELD needs the attenuation field");
+    *attf = reinterpret_cast<const R *>(table);
+    return SPECT_OK;
+  }
   if (!mu) return SPECT_OK;
   if (table) {
     *att = reinterpret_cast<const double *>(table);
@@ -203,34 +209,37 @@ int att_rows(const spect_plan *p, const Geo<R> &g, const R *mu, const void *tabl
 }
 
 template <typename R>
-int forward_impl(const spect_plan *p, const R *f, const R *mu, const void *table,
+int forward_impl(const spect_plan *p, const R *f, const R *mu, const void *table, bool field,
                  const int32_t *idx, int n, R *proj, const R *gm, const R *s, R *pred, R *ratio,
                  R delta, unsigned char *ws, hipStream_t st, bool trb = false) {
   const Geo<R> g = geo<R>(p);
   const Ws l = ws_layout(p, sizeof(R) == 8 ? SPECT_F64 : SPECT_F32, n);
   const double *att;
   bool global;
-  int rc = att_rows<R>(p, g, mu, table, idx, n, ws, l, st, &att, &global);
+  const R *attf;
+  int rc = att_rows<R>(p, g, mu, table, field, idx, n, ws, l, st, &att, &global, &attf);
   if (rc) return rc;
   R *part = reinterpret_cast<R *>(ws + l.part_off);
-  HIP_TRY(launch_forward<R>(g, tables<R>(p), f, mu, att, global, idx, n, part, trb, st));
+  HIP_TRY(launch_forward<R>(g, tables<R>(p), f, mu, att, global, attf, idx, n, part, trb, st));
   HIP_TRY(launch_forward_reduce<R>(g, part, n, proj, gm, s, pred, ratio, delta, st));
   return SPECT_OK;
 }
 
 template <typename R>
 int backward_impl(const spect_plan *p, int mode, int out_mode, const R *proj, const R *mu,
-                  const void *table, const int32_t *idx, int n, R *out, const R *f, const R *nrm,
-                  double delta, double floor_value, unsigned char *ws, hipStream_t st) {
+                  const void *table, bool field, const int32_t *idx, int n, R *out, const R *f,
+                  const R *nrm, double delta, double floor_value, unsigned char *ws,
+                  hipStream_t st) {
   const Geo<R> g = geo<R>(p);
   const Ws l = ws_layout(p, sizeof(R) == 8 ? SPECT_F64 : SPECT_F32, n);
   const double *att;
   bool global;
-  int rc = att_rows<R>(p, g, mu, table, idx, n, ws, l, st, &att, &global);
+  const R *attf;
+  int rc = att_rows<R>(p, g, mu, table, field, idx, n, ws, l, st, &att, &global, &attf);
   if (rc) return rc;
   R *W = reinterpret_cast<R *>(ws + l.w_off);
-  HIP_TRY(launch_bp_field<R>(g, tables<R>(p), proj, mu, att, global, idx, n, W, nullptr, nullptr,
-                             nullptr, st));
+  HIP_TRY(launch_bp_field<R>(g, tables<R>(p), proj, mu, att, global, attf, idx, n, W, nullptr,
+                             nullptr, nullptr, st));
   HIP_TRY(launch_bp_gather<R>(g, tables<R>(p), W, idx, n, mode, out_mode, out, f, nrm, R(delta),
                               R(floor_value), nullptr, nullptr, nullptr, st));
   return SPECT_OK;
@@ -251,13 +260,14 @@ int grad_impl(const spect_plan *p, const R *u, const R *f, const R *mu, const in
   if (grad_mu) {
     double *Wm = reinterpret_cast<double *>(ws + l.wm_off);
     double *Dsum = reinterpret_cast<double *>(ws + l.dsum_off);
-    HIP_TRY(launch_bp_field<R>(g, tables<R>(p), u, mu, att, false, idx, n, W, f, Wm, Dsum, st));
+    HIP_TRY(launch_bp_field<R>(g, tables<R>(p), u, mu, att, false, nullptr, idx, n, W, f, Wm, Dsum,
+                               st));
     HIP_TRY(launch_chunk_prefix<R>(g, Dsum, n, st));
     HIP_TRY(launch_bp_gather<R>(g, tables<R>(p), W, idx, n, 2, SPECT_OUT_STORE, grad_f, nullptr,
                                 nullptr, R(0), R(0), Wm, Dsum, grad_mu, st));
   } else {
-    HIP_TRY(launch_bp_field<R>(g, tables<R>(p), u, mu, att, false, idx, n, W, nullptr, nullptr,
-                               nullptr, st));
+    HIP_TRY(launch_bp_field<R>(g, tables<R>(p), u, mu, att, false, nullptr, idx, n, W, nullptr,
+                               nullptr, nullptr, st));
     HIP_TRY(launch_bp_gather<R>(g, tables<R>(p), W, idx, n, SPECT_BP_ADJOINT, SPECT_OUT_STORE,
                                 grad_f, nullptr, nullptr, R(0), R(0), nullptr, nullptr, nullptr, st));
   }
@@ -274,6 +284,7 @@ struct spect_batch {
   int n_subsets;
   struct Group {
     int H, Nz;
+    bool af;   // members pass precomputed attenuation factors
     int nprob;
     int maxL;  // largest chunk length among the members (LDS tap staging)
     // per subset: device offsets (bytes) into `tables` and block counts
@@ -295,7 +306,9 @@ int batch_build(spect_batch *b, const spect_batch_problem *probs, int n, double 
   std::vector<std::vector<int>> members;
   std::vector<std::pair<int, int>> keys;
   for (int i = 0; i < n; ++i) {
-    const std::pair<int, int> key{probs[i].plan->H, probs[i].plan->Nz};
+    // the key packs (H, attenuation-field flag) and Nz
+    const std::pair<int, int> key{probs[i].plan->H * 2 + (probs[i].att_field ? 1 : 0),
+                                  probs[i].plan->Nz};
     size_t k = 0;
     while (k < keys.size() && keys[k] != key) ++k;
     if (k == keys.size()) {
@@ -314,7 +327,8 @@ int batch_build(spect_batch *b, const spect_batch_problem *probs, int n, double 
   };
   for (size_t k = 0; k < keys.size(); ++k) {
     spect_batch::Group G;
-    G.H = keys[k].first;
+    G.H = keys[k].first / 2;
+    G.af = (keys[k].first & 1) != 0;
     G.Nz = keys[k].second;
     G.nprob = (int)members[k].size();
     G.maxL = 0;
@@ -338,15 +352,16 @@ int batch_build(spect_batch *b, const spect_batch_problem *probs, int n, double 
         R *W = reinterpret_cast<R *>(ws + l.w_off);
         R *ratio = reinterpret_cast<R *>(ws + l.ratio_off);
         const R *mu = reinterpret_cast<const R *>(pb.mu);
-        const double *att = reinterpret_cast<const double *>(pb.att_table);
+        const double *att = pb.att_field ? nullptr : reinterpret_cast<const double *>(pb.att_table);
+        const R *attf = pb.att_field ? reinterpret_cast<const R *>(pb.att_table) : nullptr;
         const int32_t *idx = pb.angle_idx[sub];
         R *f = reinterpret_cast<R *>(pb.f);
-        fj.push_back(FwdJob<R>{g, tb, f, mu, att, 1, idx, cnt, part});
+        fj.push_back(FwdJob<R>{g, tb, f, mu, att, 1, attf, idx, cnt, part});
         rj.push_back(ReduceJob<R>{g, part, cnt, nullptr, reinterpret_cast<const R *>(pb.g[sub]),
                                   pb.s ? reinterpret_cast<const R *>(pb.s[sub]) : nullptr,
                                   pb.pred ? reinterpret_cast<R *>(pb.pred[sub]) : nullptr, ratio,
                                   R(delta)});
-        lj.push_back(FieldJob<R>{g, tb, ratio, mu, att, 1, idx, cnt, W, nullptr, nullptr, nullptr});
+        lj.push_back(FieldJob<R>{g, tb, ratio, mu, att, 1, attf, idx, cnt, W, nullptr, nullptr, nullptr});
         gj.push_back(GatherJob<R>{g, tb, W, idx, cnt, SPECT_OUT_OSEM, f, f,
                                   reinterpret_cast<const R *>(pb.norm[sub]), R(delta),
                                   R(floor_value), nullptr, nullptr, nullptr});
@@ -381,7 +396,7 @@ int batch_launch(spect_batch *b, int n_iters, hipStream_t st) {
       for (const auto &G : b->groups) {
         HIP_TRY(launch_forward_batch<R>(reinterpret_cast<const FwdJob<R> *>(base + G.fwd_off[sub]),
                                         reinterpret_cast<const int *>(base + G.starts_off[0][sub]),
-                                        G.nprob, G.blocks[0][sub], G.H, G.Nz, G.maxL, st));
+                                        G.nprob, G.blocks[0][sub], G.H, G.Nz, G.maxL, G.af, st));
       }
       for (const auto &G : b->groups) {
         HIP_TRY(launch_reduce_batch<R>(reinterpret_cast<const ReduceJob<R> *>(base + G.red_off[sub]),
@@ -391,7 +406,7 @@ int batch_launch(spect_batch *b, int n_iters, hipStream_t st) {
       for (const auto &G : b->groups) {
         HIP_TRY(launch_field_batch<R>(reinterpret_cast<const FieldJob<R> *>(base + G.fld_off[sub]),
                                       reinterpret_cast<const int *>(base + G.starts_off[2][sub]),
-                                      G.nprob, G.blocks[2][sub], G.H, G.Nz, G.maxL, st));
+                                      G.nprob, G.blocks[2][sub], G.H, G.Nz, G.maxL, G.af, st));
       }
       for (const auto &G : b->groups) {
         HIP_TRY(launch_gather_batch<R>(reinterpret_cast<const GatherJob<R> *>(base + G.gat_off[sub]),
@@ -637,40 +652,72 @@ int spect_att_table_build(const spect_plan *p, int dtype, const void *mu, void *
   return SPECT_OK;
 }
 
+size_t spect_att_field_bytes(const spect_plan *p, int dtype) {
+  if (!p || (dtype != SPECT_F32 && dtype != SPECT_F64)) return 0;
+  const Geo<float> g = geo<float>(p);
+  return (size_t)p->A * p->P * (g.je1 - g.je0) * p->Nz * (dtype == SPECT_F64 ? 8 : 4);
+}
+
+int spect_att_field_build(const spect_plan *p, int dtype, const void *mu, void *field,
+                          void *stream) {
+  if (!p || !mu || !field) return fail(SPECT_E_ARG, "null argument");
+  if (int rc = check_arrays({mu, field})) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == SPECT_F32) {
+    HIP_TRY(launch_att_field<float>(geo<float>(p), tables<float>(p), (const float *)mu,
+                                    p->all_angles, p->A, (float *)field, st));
+  } else if (dtype == SPECT_F64) {
+    if (!p->has64) return fail(SPECT_E_ARG, "plan has no float64 tables");
+    HIP_TRY(launch_att_field<double>(geo<double>(p), tables<double>(p), (const double *)mu,
+                                     p->all_angles, p->A, (double *)field, st));
+  } else {
+    return fail(SPECT_E_ARG, "bad dtype");
+  }
+  return SPECT_OK;
+}
+
 int spect_forward(const spect_plan *p, int dtype, const void *f, const void *mu,
                   const void *att_table, const int32_t *angle_idx, int n, void *proj,
                   void *workspace, size_t workspace_bytes, void *stream) {
+  const bool field = (dtype & SPECT_ATT_FIELD) != 0;
+  dtype &= ~SPECT_ATT_FIELD;
   if (int rc = check_call(p, dtype, n, workspace, workspace_bytes)) return rc;
   if (!f || !proj || !angle_idx) return fail(SPECT_E_ARG, "null array argument");
   if (int rc = check_arrays({f, mu, att_table, proj})) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   auto *ws = reinterpret_cast<unsigned char *>(workspace);
   if (dtype == SPECT_F32)
-    return forward_impl<float>(p, (const float *)f, (const float *)mu, att_table, angle_idx, n,
-                               (float *)proj, nullptr, nullptr, nullptr, nullptr, 0.f, ws, st);
-  return forward_impl<double>(p, (const double *)f, (const double *)mu, att_table, angle_idx, n,
-                              (double *)proj, nullptr, nullptr, nullptr, nullptr, 0.0, ws, st);
+    return forward_impl<float>(p, (const float *)f, (const float *)mu, att_table, field, angle_idx,
+                               n, (float *)proj, nullptr, nullptr, nullptr, nullptr, 0.f, ws, st);
+  return forward_impl<double>(p, (const double *)f, (const double *)mu, att_table, field, angle_idx,
+                              n, (double *)proj, nullptr, nullptr, nullptr, nullptr, 0.0, ws, st);
 }
 
 int spect_backward_transpose(const spect_plan *p, int dtype, const void *v, const void *mu,
                              const void *att_table, const int32_t *angle_idx, int n, void *proj,
                              void *workspace, size_t workspace_bytes, void *stream) {
+  const bool field = (dtype & SPECT_ATT_FIELD) != 0;
+  dtype &= ~SPECT_ATT_FIELD;
   if (int rc = check_call(p, dtype, n, workspace, workspace_bytes)) return rc;
   if (!v || !proj || !angle_idx) return fail(SPECT_E_ARG, "null array argument");
   if (int rc = check_arrays({v, mu, att_table, proj})) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   auto *ws = reinterpret_cast<unsigned char *>(workspace);
   if (dtype == SPECT_F32)
-    return forward_impl<float>(p, (const float *)v, (const float *)mu, att_table, angle_idx, n,
-                               (float *)proj, nullptr, nullptr, nullptr, nullptr, 0.f, ws, st, true);
-  return forward_impl<double>(p, (const double *)v, (const double *)mu, att_table, angle_idx, n,
-                              (double *)proj, nullptr, nullptr, nullptr, nullptr, 0.0, ws, st, true);
+    return forward_impl<float>(p, (const float *)v, (const float *)mu, att_table, field, angle_idx,
+                               n, (float *)proj, nullptr, nullptr, nullptr, nullptr, 0.f, ws, st,
+                               true);
+  return forward_impl<double>(p, (const double *)v, (const double *)mu, att_table, field, angle_idx,
+                              n, (double *)proj, nullptr, nullptr, nullptr, nullptr, 0.0, ws, st,
+                              true);
 }
 
 int spect_backward(const spect_plan *p, int dtype, int mode, int out_mode, const void *proj,
                    const void *mu, const void *att_table, const int32_t *angle_idx, int n,
                    void *out, const void *f, const void *norm, double delta, double floor_value,
                    void *workspace, size_t workspace_bytes, void *stream) {
+  const bool field = (dtype & SPECT_ATT_FIELD) != 0;
+  dtype &= ~SPECT_ATT_FIELD;
   if (int rc = check_call(p, dtype, n, workspace, workspace_bytes)) return rc;
   if (!proj || !out || !angle_idx) return fail(SPECT_E_ARG, "null array argument");
   if (mode != SPECT_BP_REFERENCE && mode != SPECT_BP_ADJOINT) return fail(SPECT_E_ARG, "bad mode");
@@ -681,10 +728,10 @@ int spect_backward(const spect_plan *p, int dtype, int mode, int out_mode, const
   auto *ws = reinterpret_cast<unsigned char *>(workspace);
   if (dtype == SPECT_F32)
     return backward_impl<float>(p, mode, out_mode, (const float *)proj, (const float *)mu,
-                                att_table, angle_idx, n, (float *)out, (const float *)f,
+                                att_table, field, angle_idx, n, (float *)out, (const float *)f,
                                 (const float *)norm, delta, floor_value, ws, st);
   return backward_impl<double>(p, mode, out_mode, (const double *)proj, (const double *)mu,
-                               att_table, angle_idx, n, (double *)out, (const double *)f,
+                               att_table, field, angle_idx, n, (double *)out, (const double *)f,
                                (const double *)norm, delta, floor_value, ws, st);
 }
 
@@ -693,6 +740,8 @@ int spect_osem_subiteration(const spect_plan *p, int dtype, void *f, const void 
                             const void *s, const void *norm, void *pred, double delta,
                             double floor_value, void *workspace, size_t workspace_bytes,
                             void *stream) {
+  const bool field = (dtype & SPECT_ATT_FIELD) != 0;
+  dtype &= ~SPECT_ATT_FIELD;
   if (int rc = check_call(p, dtype, n, workspace, workspace_bytes)) return rc;
   if (!f || !g || !norm || !angle_idx) return fail(SPECT_E_ARG, "null array argument");
   if (int rc = check_arrays({f, mu, att_table, g, s, norm, pred})) return rc;
@@ -701,21 +750,21 @@ int spect_osem_subiteration(const spect_plan *p, int dtype, void *f, const void 
   const Ws l = ws_layout(p, dtype, n);
   if (dtype == SPECT_F32) {
     float *ratio = reinterpret_cast<float *>(ws + l.ratio_off);
-    int rc = forward_impl<float>(p, (const float *)f, (const float *)mu, att_table, angle_idx, n,
-                                 nullptr, (const float *)g, (const float *)s, (float *)pred, ratio,
-                                 (float)delta, ws, st);
+    int rc = forward_impl<float>(p, (const float *)f, (const float *)mu, att_table, field,
+                                 angle_idx, n, nullptr, (const float *)g, (const float *)s,
+                                 (float *)pred, ratio, (float)delta, ws, st);
     if (rc) return rc;
     return backward_impl<float>(p, SPECT_BP_REFERENCE, SPECT_OUT_OSEM, ratio, (const float *)mu,
-                                att_table, angle_idx, n, (float *)f, (const float *)f,
+                                att_table, field, angle_idx, n, (float *)f, (const float *)f,
                                 (const float *)norm, delta, floor_value, ws, st);
   }
   double *ratio = reinterpret_cast<double *>(ws + l.ratio_off);
-  int rc = forward_impl<double>(p, (const double *)f, (const double *)mu, att_table, angle_idx, n,
-                                nullptr, (const double *)g, (const double *)s, (double *)pred,
-                                ratio, delta, ws, st);
+  int rc = forward_impl<double>(p, (const double *)f, (const double *)mu, att_table, field,
+                                angle_idx, n, nullptr, (const double *)g, (const double *)s,
+                                (double *)pred, ratio, delta, ws, st);
   if (rc) return rc;
   return backward_impl<double>(p, SPECT_BP_REFERENCE, SPECT_OUT_OSEM, ratio, (const double *)mu,
-                               att_table, angle_idx, n, (double *)f, (const double *)f,
+                               att_table, field, angle_idx, n, (double *)f, (const double *)f,
                                (const double *)norm, delta, floor_value, ws, st);
 }
 

@@ -214,6 +214,25 @@ __device__ __forceinline__ void load_zv(const R *__restrict__ p, int z0, int Nz,
   }
 }
 
+// RZ consecutive z of one row starting at z0 (16-byte loads when the run is
+// complete and Nz is a multiple of 4, else element loads within Nz).
+template <typename R, int RZ>
+__device__ __forceinline__ void load_run(const R *__restrict__ p, int z0, int Nz, R (&v)[RZ]) {
+  constexpr int ZV = 16 / sizeof(R);
+  if (z0 + RZ <= Nz && (Nz % 4) == 0) {
+#pragma unroll
+    for (int m = 0; m < RZ; m += ZV) {
+      R t[ZV];
+      load_zv<R, ZV>(p, z0 + m, Nz, true, t);
+#pragma unroll
+      for (int e = 0; e < ZV; ++e) v[m + e] = t[e];
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < RZ; ++m) v[m] = z0 + m < Nz ? p[z0 + m] : R(0);
+  }
+}
+
 // Bilinear sample of ZV consecutive z, in interp()'s order.
 template <typename R, int ZV>
 __device__ __forceinline__ void interp_zv(const R *__restrict__ vol, const Taps4<R> &t, int z0,

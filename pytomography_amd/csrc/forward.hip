@@ -90,6 +90,57 @@ __global__ __launch_bounds__(1024) void att_table_kernel(Geo<R> g, Tables<R> tb,
   }
 }
 
+// The attenuation factor itself, a(i, j, z) = exp(-(S(i) - x(i)/2)) in R, for
+// every plane whose tile footprint is non-empty (the planes the projection
+// kernels visit), from the same FP64 march as att_table_kernel: the forward
+// and field kernels then read it instead of gathering mu (bitwise the same
+// factors).  field layout: [angle][P][je1-je0][Nz].
+template <typename R, int RZ>
+__global__ __launch_bounds__(1024) void att_field_kernel(Geo<R> g, Tables<R> tb,
+                                                        const R *__restrict__ mu,
+                                                        const int32_t *__restrict__ angle_idx,
+                                                        R *__restrict__ field) {
+  __shared__ Src<R> src[kTJ];
+  const int al = blockIdx.x, jt = blockIdx.y;
+  const int ang = ld_uniform(angle_idx + al);
+  if ((unsigned)ang >= (unsigned)g.A) return;
+  const int tid = threadIdx.x;
+  const int j0 = g.je0 + jt * kTJ;
+  const int tj = min(kTJ, g.je1 - j0);
+  const int runs = (g.Nz + RZ - 1) / RZ;
+  const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
+  const bool mine = run < runs && jl < tj;
+  const int n_jext = g.je1 - g.je0;
+  const auto cs = ld_uniform(tb.trig + ang);
+  double S[RZ];
+#pragma unroll
+  for (int m = 0; m < RZ; ++m) S[m] = 0.0;
+  for (int i = g.P - 1; i >= 0; --i) {
+    int inside = 0;
+    if (tid < tj) {
+      R ix, iy;
+      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], ld_uniform(tb.lin + i), g.half, ix, iy);
+      src[tid] = make_src<R>(ix, iy, g.pad);
+      inside = in_footprint<R>(ix, iy, g);
+    }
+    if (__syncthreads_or(inside)) {
+      if (mine) {
+        const Taps4<R> t = taps4<R>(src[jl], g.N, g.Nz);
+        R x[RZ];
+        gather_run<R, RZ>(mu, t, z0, g.Nz, x);
+        R *dst = field + (((size_t)al * g.P + i) * n_jext + (j0 - g.je0 + jl)) * g.Nz + z0;
+#pragma unroll
+        for (int m = 0; m < RZ; ++m) {
+          const R xm = x[m] * g.dx;
+          S[m] += (double)xm;
+          if (z0 + m < g.Nz) dst[m] = att_factor<R>(S[m], xm);
+        }
+      }
+      __syncthreads();  // src is rewritten by the next plane
+    }
+  }
+}
+
 // LDS rows: H zeros | runs*RZ values | H zeros | 1 pad (odd stride).
 template <int RZ, int H> __device__ __forceinline__ int lds_stride(int runs) {
   return runs * RZ + 2 * H + 1;
@@ -102,7 +153,7 @@ template <int RZ, int H> __device__ __forceinline__ int lds_stride(int runs) {
 // that tap's weight -- the body then computes B^T v (attenuation, PSF, sum).
 constexpr int kTrbMax = 12;  // contributors per rotated-frame pixel (<= 9 in exact arithmetic)
 
-template <typename R, int RZ, int QS, int H, bool TRB>
+template <typename R, int RZ, int QS, int H, bool TRB, bool AF>
 __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int c,
                                          unsigned char *smem) {
   constexpr int K = 2 * H + 1;
@@ -121,6 +172,7 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   const R *__restrict__ mu = J.mu;
   const double *__restrict__ att = J.att;
   const int att_global = J.att_global;
+  const R *__restrict__ attf = J.attf;  // AF: precomputed attenuation factors
   const int32_t *__restrict__ angle_idx = J.angle_idx;
   R *__restrict__ partial = J.partial;
   const int n_call = J.n;
@@ -159,14 +211,14 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   // V and Vz start zeroed: halo pads, columns >= tj and z slots >= Nz are
   // never written afterwards
   for (int q = tid; q < 2 * kTJ * ldv; q += blockDim.x) V[q] = R(0);
-  double S[NPA][ZV];
+  double S[AF ? 1 : NPA][ZV];
 #pragma unroll
-  for (int p = 0; p < NPA; ++p) {
+  for (int p = 0; p < (AF ? 1 : NPA); ++p) {
     const int jj = aco + p * cpp;
 #pragma unroll
     for (int e = 0; e < ZV; ++e) {
       S[p][e] = 0.0;
-      if (has_mu && aact && jj < tj && za + e < Nz) {
+      if (!AF && has_mu && aact && jj < tj && za + e < Nz) {
         const int rowa = att_global ? ang : al;
         S[p][e] = att[(((size_t)rowa * g.C + c) * n_jext + (j0 - g.je0 + jj)) * Nz + za + e];
       }
@@ -277,14 +329,20 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
           } else {
             interp_zv<R, ZV>(f, t, za, Nz, zvec, val);
           }
-          if (has_mu) {
+          if constexpr (AF) {
+            R a[ZV];
+            load_zv<R, ZV>(attf + (((size_t)ang * g.P + i) * n_jext + (j0 - g.je0 + jj)) * Nz, za,
+                           Nz, zvec, a);
+#pragma unroll
+            for (int e = 0; e < ZV; ++e) val[e] = val[e] * a[e];
+          } else if (has_mu) {
             R x[ZV];
             interp_zv<R, ZV>(mu, t, za, Nz, zvec, x);
 #pragma unroll
             for (int e = 0; e < ZV; ++e) {
               const R xm = x[e] * g.dx;
-              S[p][e] += (double)xm;
-              val[e] = val[e] * att_factor<R>(S[p][e], xm);
+              S[AF ? 0 : p][e] += (double)xm;
+              val[e] = val[e] * att_factor<R>(S[AF ? 0 : p][e], xm);
             }
           }
 #pragma unroll
@@ -373,15 +431,15 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   }
 }
 
-template <typename R, int RZ, int QS, int H, bool TRB>
+template <typename R, int RZ, int QS, int H, bool TRB, bool AF>
 __global__ __launch_bounds__(512) void fwd_kernel(FwdJob<R> J) {
   extern __shared__ __align__(16) unsigned char smem[];
-  fwd_body<R, RZ, QS, H, TRB>(J, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+  fwd_body<R, RZ, QS, H, TRB, AF>(J, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 
 // Batched: block b belongs to job find_job(b); within a job the blocks run
 // over (angle, tile, chunk) with the chunk fastest.
-template <typename R, int RZ, int QS, int H>
+template <typename R, int RZ, int QS, int H, bool AF>
 __global__ __launch_bounds__(512) void fwd_batch_kernel(const FwdJob<R> *__restrict__ jobs,
                                                         const int *__restrict__ starts, int njobs) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -390,7 +448,7 @@ __global__ __launch_bounds__(512) void fwd_batch_kernel(const FwdJob<R> *__restr
   const FwdJob<R> J = ld_uniform_struct(jobs + j);
   const int local = b - ld_uniform(starts + j);
   const int c = local % J.g.C, r = local / J.g.C;
-  fwd_body<R, RZ, QS, H, false>(J, r / J.g.ntiles, r % J.g.ntiles, c, smem);
+  fwd_body<R, RZ, QS, H, false, AF>(J, r / J.g.ntiles, r % J.g.ntiles, c, smem);
 }
 
 template <typename R>
@@ -449,21 +507,22 @@ __global__ __launch_bounds__(256) void reduce_batch_kernel(const ReduceJob<R> *_
 
 template <typename R, int H>
 hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *mu,
-                        const double *att, bool att_global, const int32_t *idx, int n, R *partial,
-                        bool trb, hipStream_t st) {
+                        const double *att, bool att_global, const R *attf, const int32_t *idx,
+                        int n, R *partial, bool trb, hipStream_t st) {
   constexpr int RZ = FwdTile<R>::RZ, QS = FwdTile<R>::QS;
   const int bt = tile_block_threads(g.Nz, RZ, QS);
   if (bt > 512) return hipErrorInvalidConfiguration;  // __launch_bounds__(512)
   if (!phase_a_fits<R, QS>(bt, g.Nz)) return hipErrorInvalidConfiguration;
   const int runs = (g.Nz + RZ - 1) / RZ;
   const size_t smem = (2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)g.L * 2 * (2 * H + 1)) * sizeof(R);
-  auto kern = trb ? fwd_kernel<R, RZ, QS, H, true> : fwd_kernel<R, RZ, QS, H, false>;
+  auto kern = attf ? (trb ? fwd_kernel<R, RZ, QS, H, true, true> : fwd_kernel<R, RZ, QS, H, false, true>)
+                   : (trb ? fwd_kernel<R, RZ, QS, H, true, false> : fwd_kernel<R, RZ, QS, H, false, false>);
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
   }
-  FwdJob<R> J{g, tb, f, mu, att, att_global ? 1 : 0, idx, n, partial};
+  FwdJob<R> J{g, tb, f, mu, att, att_global ? 1 : 0, attf, idx, n, partial};
   dim3 grid(n, g.ntiles, g.C);
   kern<<<grid, bt, smem, st>>>(J);
   return hipGetLastError();
@@ -471,14 +530,14 @@ hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const 
 
 template <typename R, int H>
 hipError_t launch_fwd_batch_h(const FwdJob<R> *jobs, const int *starts, int njobs, int blocks,
-                              int Nz, int max_L, hipStream_t st) {
+                              int Nz, int max_L, bool af, hipStream_t st) {
   constexpr int RZ = FwdTile<R>::RZ, QS = FwdTile<R>::QS;
   const int bt = tile_block_threads(Nz, RZ, QS);
   if (bt > 512) return hipErrorInvalidConfiguration;  // __launch_bounds__(512)
   if (!phase_a_fits<R, QS>(bt, Nz)) return hipErrorInvalidConfiguration;
   const int runs = (Nz + RZ - 1) / RZ;
   const size_t smem = (2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)max_L * 2 * (2 * H + 1)) * sizeof(R);
-  auto kern = fwd_batch_kernel<R, RZ, QS, H>;
+  auto kern = af ? fwd_batch_kernel<R, RZ, QS, H, true> : fwd_batch_kernel<R, RZ, QS, H, false>;
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
@@ -502,11 +561,22 @@ hipError_t launch_att_table(const Geo<R> &g, const Tables<R> &tb, const R *mu,
 }
 
 template <typename R>
+hipError_t launch_att_field(const Geo<R> &g, const Tables<R> &tb, const R *mu,
+                            const int32_t *angle_idx, int n, R *field, hipStream_t st) {
+  constexpr int RZ = FwdTile<R>::RZ;
+  const int runs = (g.Nz + RZ - 1) / RZ;
+  const int bt = (kTJ * runs + 63) / 64 * 64;
+  dim3 grid(n, g.ntiles);
+  att_field_kernel<R, RZ><<<grid, bt, 0, st>>>(g, tb, mu, angle_idx, field);
+  return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_forward(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *mu,
-                          const double *att, bool att_global, const int32_t *idx, int n,
-                          R *partial, bool trb, hipStream_t st) {
+                          const double *att, bool att_global, const R *attf, const int32_t *idx,
+                          int n, R *partial, bool trb, hipStream_t st) {
 #define SPECT_F_CASE(HH) \
-  case HH: return launch_fwd_h<R, HH>(g, tb, f, mu, att, att_global, idx, n, partial, trb, st);
+  case HH: return launch_fwd_h<R, HH>(g, tb, f, mu, att, att_global, attf, idx, n, partial, trb, st);
   switch (g.H) {
     SPECT_F_CASE(0) SPECT_F_CASE(2) SPECT_F_CASE(4) SPECT_F_CASE(6) SPECT_F_CASE(8)
     SPECT_F_CASE(10) SPECT_F_CASE(12) SPECT_F_CASE(14) SPECT_F_CASE(16) SPECT_F_CASE(18)
@@ -526,9 +596,9 @@ hipError_t launch_forward_reduce(const Geo<R> &g, const R *partial, int n, R *pr
 
 template <typename R>
 hipError_t launch_forward_batch(const FwdJob<R> *jobs, const int *starts, int njobs, int blocks,
-                                int H, int Nz, int max_L, hipStream_t st) {
+                                int H, int Nz, int max_L, bool af, hipStream_t st) {
 #define SPECT_FB_CASE(HH) \
-  case HH: return launch_fwd_batch_h<R, HH>(jobs, starts, njobs, blocks, Nz, max_L, st);
+  case HH: return launch_fwd_batch_h<R, HH>(jobs, starts, njobs, blocks, Nz, max_L, af, st);
   switch (H) {
     SPECT_FB_CASE(0) SPECT_FB_CASE(2) SPECT_FB_CASE(4) SPECT_FB_CASE(6) SPECT_FB_CASE(8)
     SPECT_FB_CASE(10) SPECT_FB_CASE(12) SPECT_FB_CASE(14) SPECT_FB_CASE(16) SPECT_FB_CASE(18)
@@ -548,13 +618,15 @@ hipError_t launch_reduce_batch(const ReduceJob<R> *jobs, const int *starts, int 
 #define SPECT_INST(R)                                                                          \
   template hipError_t launch_att_table<R>(const Geo<R> &, const Tables<R> &, const R *,        \
                                           const int32_t *, int, double *, hipStream_t);        \
+  template hipError_t launch_att_field<R>(const Geo<R> &, const Tables<R> &, const R *,        \
+                                          const int32_t *, int, R *, hipStream_t);             \
   template hipError_t launch_forward<R>(const Geo<R> &, const Tables<R> &, const R *,          \
-                                        const R *, const double *, bool, const int32_t *, int, \
-                                        R *, bool, hipStream_t);                               \
+                                        const R *, const double *, bool, const R *,            \
+                                        const int32_t *, int, R *, bool, hipStream_t);         \
   template hipError_t launch_forward_reduce<R>(const Geo<R> &, const R *, int, R *, const R *, \
                                                const R *, R *, R *, R, hipStream_t);           \
   template hipError_t launch_forward_batch<R>(const FwdJob<R> *, const int *, int, int, int,    \
-                                              int, int, hipStream_t);                               \
+                                              int, int, bool, hipStream_t);                    \
   template hipError_t launch_reduce_batch<R>(const ReduceJob<R> *, const int *, int, int,       \
                                              hipStream_t);
 SPECT_INST(float)

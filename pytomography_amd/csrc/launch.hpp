@@ -20,6 +20,7 @@ template <typename R> struct FwdJob {
   const R *mu;
   const double *att;
   int att_global;
+  const R *attf;  // precomputed attenuation factors [A][P][je1-je0][Nz] (or null)
   const int32_t *angle_idx;
   int n;
   R *partial;
@@ -44,6 +45,7 @@ template <typename R> struct FieldJob {
   const R *mu;
   const double *att;
   int att_global;
+  const R *attf;  // precomputed attenuation factors [A][P][je1-je0][Nz] (or null)
   const int32_t *angle_idx;
   int n;
   R *W;
@@ -83,13 +85,13 @@ template <typename R> int gather_blocks(const Geo<R> &g) {
 // are device arrays; all jobs of one launch share the PSF half width H and Nz.
 template <typename R>
 hipError_t launch_forward_batch(const FwdJob<R> *jobs, const int *starts, int njobs, int blocks,
-                                int H, int Nz, int max_L, hipStream_t st);
+                                int H, int Nz, int max_L, bool af, hipStream_t st);
 template <typename R>
 hipError_t launch_reduce_batch(const ReduceJob<R> *jobs, const int *starts, int njobs, int blocks,
                                hipStream_t st);
 template <typename R>
 hipError_t launch_field_batch(const FieldJob<R> *jobs, const int *starts, int njobs, int blocks,
-                              int H, int Nz, int max_L, hipStream_t st);
+                              int H, int Nz, int max_L, bool af, hipStream_t st);
 template <typename R>
 hipError_t launch_gather_batch(const GatherJob<R> *jobs, const int *starts, int njobs, int blocks,
                                int mode, hipStream_t st);
@@ -104,13 +106,18 @@ template <typename R>
 hipError_t launch_att_table(const Geo<R> &g, const Tables<R> &tb, const R *mu,
                             const int32_t *angle_idx, int n, double *table, hipStream_t st);
 
+// Attenuation factors of every visited plane: field[n][P][je1-je0][Nz] (R).
+template <typename R>
+hipError_t launch_att_field(const Geo<R> &g, const Tables<R> &tb, const R *mu,
+                            const int32_t *angle_idx, int n, R *field, hipStream_t st);
+
 // Forward projection partials: partial[C][n][ntiles][kTJ + 2H][Nz].  With trb
 // the rotation is the transpose of the reference back projection's gather, so
 // the result is B^T f (B = spect_backward REFERENCE).
 template <typename R>
 hipError_t launch_forward(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *mu,
-                          const double *att, bool att_global, const int32_t *angle_idx, int n,
-                          R *partial, bool trb, hipStream_t st);
+                          const double *att, bool att_global, const R *attf,
+                          const int32_t *angle_idx, int n, R *partial, bool trb, hipStream_t st);
 
 // proj[n][jo1-jo0][Nz] = sum of the partials; if g_meas != nullptr also
 // pred = proj + s (s may be null) and ratio = g_meas / (pred + delta).
@@ -124,8 +131,9 @@ hipError_t launch_forward_reduce(const Geo<R> &g, const R *partial, int n, R *pr
 // chunk totals Dsum[n][C][je1-je0][Nz], both FP64.
 template <typename R>
 hipError_t launch_bp_field(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
-                           const double *att, bool att_global, const int32_t *angle_idx, int n,
-                           R *W, const R *f, double *Wm, double *Dsum, hipStream_t st);
+                           const double *att, bool att_global, const R *attf,
+                           const int32_t *angle_idx, int n, R *W, const R *f, double *Wm,
+                           double *Dsum, hipStream_t st);
 
 // Dsum <- prefix over chunks (in place).
 template <typename R>

@@ -45,9 +45,19 @@ class Likelihood:
         self.n_subsets_previous = n_subsets
 
     def _get_projection_subset(self, projections: torch.Tensor, subset_idx: int | None = None):
+        """likelihood.py:58-72.  The measured projections and the additive term
+        do not change between iterations, so each subset's contiguous copy is
+        kept until the tensor is modified (its version counter moves)."""
         if subset_idx is None:
             return projections
-        return self.system_matrix.get_projection_subset(projections, subset_idx)
+        cache = self.__dict__.setdefault("_subset_cache", {})
+        key = (id(projections), subset_idx)
+        hit = cache.get(key)
+        if hit is not None and hit[0] is projections and hit[1] == projections._version:
+            return hit[2]
+        sub = self.system_matrix.get_projection_subset(projections, subset_idx).contiguous()
+        cache[key] = (projections, projections._version, sub)
+        return sub
 
     def _get_normBP(self, subset_idx, return_sum: bool = False):
         """likelihood.py:74-91."""

@@ -69,11 +69,19 @@ class SPECTSystemMatrix(SystemMatrix):
         proj_meta: ``SPECTProjMeta``.
         object_initial_based_on_camera_path: not supported (must be False).
         sensitivity_dtype: dtype of the H^T 1 computation (float64 default).
+        chunk_planes: planes per kernel work unit (0: library default).
+        attenuation_field: keep the attenuation factor of every rotated-frame
+            voxel of every angle resident in device memory (built once per
+            attenuation map) instead of resampling mu in every projection.
+            None: on for float32 when it needs at most ``ATT_FIELD_AUTO_BYTES``.
     """
+
+    ATT_FIELD_AUTO_BYTES = 8 << 30
 
     def __init__(self, obj2obj_transforms, proj2proj_transforms, object_meta, proj_meta,
                  object_initial_based_on_camera_path: bool = False,
-                 sensitivity_dtype: torch.dtype = torch.float64, chunk_planes: int = 0) -> None:
+                 sensitivity_dtype: torch.dtype = torch.float64, chunk_planes: int = 0,
+                 attenuation_field: bool | None = None) -> None:
         super().__init__(object_meta, proj_meta, obj2obj_transforms, proj2proj_transforms)
         if object_initial_based_on_camera_path:
             raise NotImplementedError("object_initial_based_on_camera_path is not supported")
@@ -88,6 +96,7 @@ class SPECTSystemMatrix(SystemMatrix):
                              f"{(shape[1], shape[2])}")
         self._shape = shape
         self.chunk_planes = int(chunk_planes)
+        self.attenuation_field = attenuation_field
         self._plans: dict = {}
         self._ws: dict = {}
         self._idx32: dict = {}
@@ -200,8 +209,17 @@ class SPECTSystemMatrix(SystemMatrix):
             self._att_tables.pop(key, None)
         return self._mu_cache[key][1]
 
+    def _use_field(self, device, dt) -> bool:
+        if self.attenuation_field is not None:
+            return bool(self.attenuation_field)
+        if dt != torch.float32:
+            return False
+        plan = self._plan(device)
+        return int(plan.lib.spect_att_field_bytes(plan.handle, _lib.dtype_code(dt))) <= self.ATT_FIELD_AUTO_BYTES
+
     def _att_table(self, device, dt):
-        """FP64 attenuation suffix-sum table of the current mu (built once per map)."""
+        """Attenuation data of the current mu, built once per map: the FP64
+        suffix-sum table, or the attenuation field (see ``attenuation_field``)."""
         mu = self._mu(device, dt)
         if mu is None:
             return None
@@ -209,12 +227,26 @@ class SPECTSystemMatrix(SystemMatrix):
         tab = self._att_tables.get(key)
         if tab is None:
             plan = self._plan(device)
-            nbytes = int(plan.lib.spect_att_table_bytes(plan.handle))
-            tab = torch.empty(nbytes // 8, dtype=torch.float64, device=device)
-            _lib.check(plan.lib.spect_att_table_build(plan.handle, _lib.dtype_code(dt), mu.data_ptr(),
-                                                      tab.data_ptr(), _lib.stream_ptr(device)))
+            if self._use_field(device, dt):
+                nbytes = int(plan.lib.spect_att_field_bytes(plan.handle, _lib.dtype_code(dt)))
+                tab = torch.empty(nbytes // (torch.finfo(dt).bits // 8), dtype=dt, device=device)
+                _lib.check(plan.lib.spect_att_field_build(plan.handle, _lib.dtype_code(dt), mu.data_ptr(),
+                                                          tab.data_ptr(), _lib.stream_ptr(device)))
+                tab.att_field = True
+            else:
+                nbytes = int(plan.lib.spect_att_table_bytes(plan.handle))
+                tab = torch.empty(nbytes // 8, dtype=torch.float64, device=device)
+                _lib.check(plan.lib.spect_att_table_build(plan.handle, _lib.dtype_code(dt), mu.data_ptr(),
+                                                          tab.data_ptr(), _lib.stream_ptr(device)))
+                tab.att_field = False
             self._att_tables[key] = tab
         return tab
+
+    @staticmethod
+    def _code(dt, tab) -> int:
+        """dtype argument of the projection entry points for attenuation data ``tab``."""
+        code = _lib.dtype_code(dt)
+        return code | _lib.ATT_FIELD if tab is not None and tab.att_field else code
 
     @staticmethod
     def _compute_dtype(t: torch.Tensor) -> torch.dtype:
@@ -271,7 +303,7 @@ class SPECTSystemMatrix(SystemMatrix):
         tab = self._att_table(device, dt)
         out = torch.empty((n, self._shape[1], self._shape[2]), dtype=dt, device=device)
         ws, nbytes = self._workspace(device, dt, n)
-        _lib.check(getattr(plan.lib, entry)(plan.handle, _lib.dtype_code(dt), f.data_ptr(),
+        _lib.check(getattr(plan.lib, entry)(plan.handle, self._code(dt, tab), f.data_ptr(),
                                             _lib.ptr(mu), _lib.ptr(tab), idx.data_ptr(), n,
                                             out.data_ptr(), ws.data_ptr(), nbytes,
                                             _lib.stream_ptr(device)))
@@ -293,7 +325,7 @@ class SPECTSystemMatrix(SystemMatrix):
             out = torch.empty(self._shape, dtype=dt, device=device)
         ws, nbytes = self._workspace(device, dt, n)
         _lib.check(plan.lib.spect_backward(
-            plan.handle, _lib.dtype_code(dt), mode, out_mode, p.data_ptr(), _lib.ptr(mu),
+            plan.handle, self._code(dt, tab), mode, out_mode, p.data_ptr(), _lib.ptr(mu),
             _lib.ptr(tab), idx.data_ptr(), n, out.data_ptr(), _lib.ptr(f), _lib.ptr(norm),
             float(pytomography_amd.delta), float(floor_value), ws.data_ptr(), nbytes,
             _lib.stream_ptr(device)))
@@ -331,7 +363,7 @@ class SPECTSystemMatrix(SystemMatrix):
             pred_out = torch.empty((n, self._shape[1], self._shape[2]), dtype=dt, device=device)
         ws, nbytes = self._workspace(device, dt, n)
         _lib.check(plan.lib.spect_osem_subiteration(
-            plan.handle, _lib.dtype_code(dt), object.data_ptr(), _lib.ptr(mu), _lib.ptr(tab),
+            plan.handle, self._code(dt, tab), object.data_ptr(), _lib.ptr(mu), _lib.ptr(tab),
             idx.data_ptr(), n, g.data_ptr(), _lib.ptr(s), nrm.data_ptr(),
             _lib.ptr(pred_out) if keep_prediction else None, float(pytomography_amd.delta),
             float(floor_value), ws.data_ptr(), nbytes, _lib.stream_ptr(device)))

@@ -23,7 +23,7 @@ def oracle_geometry(pa, dtype=torch.float32):
                             radii=pa["radii"], mu=mu, psf=psf, dtype=dtype)
 
 
-def product_system_matrix(pa, device="cuda", chunk_planes=0):
+def product_system_matrix(pa, device="cuda", chunk_planes=0, attenuation_field=None):
     import pytomography_amd
     from pytomography_amd.metadata.SPECT import SPECTObjectMeta, SPECTProjMeta, SPECTPSFMeta
     from pytomography_amd.transforms.SPECT import SPECTAttenuationTransform, SPECTPSFTransform
@@ -37,7 +37,8 @@ def product_system_matrix(pa, device="cuda", chunk_planes=0):
         tr.append(SPECTAttenuationTransform(torch.from_numpy(pa["mu"]).to(device)))
     if pa["psf"]:
         tr.append(SPECTPSFTransform(SPECTPSFMeta(synthetic.PSF_PARAMS)))
-    return SPECTSystemMatrix(tr, [], om, pm, chunk_planes=chunk_planes)
+    return SPECTSystemMatrix(tr, [], om, pm, chunk_planes=chunk_planes,
+                             attenuation_field=attenuation_field)
 
 
 def rel_l2(a, b):

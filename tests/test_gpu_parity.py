@@ -233,3 +233,28 @@ def test_bsrem_iterates_match_oracle(cuda):
     errs = [rel_l2(a, b) for a, b in zip(cb.iterates, ref_iterates)]
     print("BSREM+RDP per-iterate rel-L2: " + " ".join(f"{e:.1e}" for e in errs))
     assert len(errs) == iters * subsets and max(errs) <= TOL
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_attenuation_field_matches_suffix_table(cuda, dt):
+    """The resident attenuation field (SPECT_ATT_FIELD) and the FP64 suffix
+    table give the same projections: both evaluate the same FP64 march."""
+    pa = problem_arrays(33, 10, seed=15, nz=20, att=True, psf=True)
+    a = product_system_matrix(pa, attenuation_field=True)
+    b = product_system_matrix(pa, attenuation_field=False)
+    a.set_n_subsets(2)
+    b.set_n_subsets(2)
+    gen = torch.Generator().manual_seed(16)
+    f = torch.rand((33, 33, 20), generator=gen, dtype=dt).to(cuda)
+    y = torch.rand((5, 33, 20), generator=gen, dtype=dt).to(cuda)
+    pairs = [
+        ("forward", a.forward(f, 1), b.forward(f, 1)),
+        ("backward", a.backward(y, 1), b.backward(y, 1)),
+        ("backward_adjoint", a.backward_adjoint(y, 1), b.backward_adjoint(y, 1)),
+        ("backward_transpose", a.backward_transpose(f, 1), b.backward_transpose(f, 1)),
+    ]
+    assert a._att_table(f.device, dt).att_field and not b._att_table(f.device, dt).att_field
+    for name, x, z in pairs:
+        e = rel_l2(x, z)
+        print(f"{dt} {name}: field vs suffix table rel-L2 {e:.1e}, max |diff| {float((x - z).abs().max()):.1e}")
+        assert e <= (1e-6 if dt == torch.float32 else 1e-14)
