@@ -302,6 +302,24 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
     if (tid < tj) setup(i, 0);
   }
   __syncthreads();
+  // PF: the attenuation factors of the next plane (a coalesced stream from
+  // HBM) are loaded into registers right after the phase-A barrier, so their
+  // latency hides behind the blur phases of the current plane
+  constexpr bool PF = AF;
+  R Ar[PF ? NPA : 1][ZV];
+  auto issue = [&](int ip, int bb) {
+    if (!aact) return;
+#pragma unroll
+    for (int p = 0; p < NPA; ++p) {
+      const int jj = aco + p * cpp;
+      if (jj < tj)
+        load_zv<R, ZV>(attf + (((size_t)ang * g.P + ip) * n_jext + (j0 - g.je0 + jj)) * Nz, za, Nz,
+                       zvec, Ar[PF ? p : 0]);
+    }
+  };
+  if constexpr (PF) {
+    if (i >= 0) issue(i, 0);
+  }
   int b = 0;
   while (i >= 0) {
     const int inext = rem ? i_lo + 63 - __clzll(rem) : -1;
@@ -329,7 +347,10 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
           } else {
             interp_zv<R, ZV>(f, t, za, Nz, zvec, val);
           }
-          if constexpr (AF) {
+          if constexpr (PF) {
+#pragma unroll
+            for (int e = 0; e < ZV; ++e) val[e] = val[e] * Ar[p][e];
+          } else if constexpr (AF) {
             R a[ZV];
             load_zv<R, ZV>(attf + (((size_t)ang * g.P + i) * n_jext + (j0 - g.je0 + jj)) * Nz, za,
                            Nz, zvec, a);
@@ -354,6 +375,9 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
     if (inext >= 0 && tid < tj) setup(inext, b ^ 1);
     if constexpr (H == 0) {
       __syncthreads();
+      if constexpr (PF) {
+        if (inext >= 0) issue(inext, b ^ 1);
+      }
       if (cact) {
         // no PSF: the ray sum of column jl is the plain sum over planes
 #pragma unroll
@@ -365,6 +389,9 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
       __syncthreads();
     } else {
       __syncthreads();
+      if constexpr (PF) {
+        if (inext >= 0) issue(inext, b ^ 1);
+      }
       // ---- B: z blur (correlation, zero padded) of the own run
       const R *wz = Wt + (size_t)(i - i_lo) * 2 * K;
       if (ab) {
