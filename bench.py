@@ -324,6 +324,14 @@ def main():
         return
     att = psf = args.workload == "att_psf"
     n, na, ns = args.n, args.angles, args.subsets
+    if not att and n * n * n * na > 64 * 64 * 64 * 60:
+        # A run of this workload at 128^3 x 120 ended in torch.poisson's
+        # non-negative-rate device assert while simulating the data; the
+        # cause is not identified, so it stays off the GPU at that size.
+        print("bench.py: --workload plain above 64^3 x 60 is disabled (unresolved abort while "
+              "simulating the data; see the known-issues section of the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
 
     from pytomography_amd.algorithms import OSEM
     from pytomography_amd.likelihoods import PoissonLogLikelihood
