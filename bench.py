@@ -229,6 +229,7 @@ def bench_unrolled(args, device):
         x0 = torch.ones(sm._shape, dtype=dt, device=device).requires_grad_(True)
         beta = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
         un._norms(dt, device)                      # sensitivities, one-off
+        sm._att_table(device, dt)                  # attenuation data of the map, one-off
         torch.cuda.synchronize()
         torch.cuda.reset_peak_memory_stats(device)
         base = torch.cuda.memory_allocated(device)
