@@ -37,3 +37,43 @@ class PoissonLogLikelihood(Likelihood):
         self.projections_predicted, ratio = self.poisson_ratio(proj_subset, hf, additive_term_subset)
         norm_BP = self._get_normBP(subset_idx)
         return self.system_matrix.backward(ratio, subset_idx) - norm_BP
+
+    # ---------------------------------------------------------- second order
+    # Operators of poisson_log_likelihood.py:38-114, built on the accelerated
+    # forward / reference back projection.  FP is H_m f (the additive term is
+    # not included, as in the reference); note the three different placements
+    # of delta.
+    def _fp(self, object, precomputed_forward_projection, subset_idx):
+        if precomputed_forward_projection is None:
+            return self.system_matrix.forward(object, subset_idx)
+        return precomputed_forward_projection
+
+    def compute_gradient_ff(self, object, precomputed_forward_projection=None, subset_idx=None):
+        """x -> -H^T (g / (FP^2 + delta) * H x)  (poisson_log_likelihood.py:38-61)."""
+        FP = self._fp(object, precomputed_forward_projection, subset_idx)
+        g = self._get_projection_subset(self.projections, subset_idx)
+        weight = g / (FP ** 2 + pytomography_amd.delta)
+
+        def operator(x):
+            return -self.system_matrix.backward(self.system_matrix.forward(x, subset_idx) * weight,
+                                                subset_idx)
+        return operator
+
+    def compute_gradient_gf(self, object, precomputed_forward_projection=None, subset_idx=None):
+        """x -> H x / (FP + delta)  (poisson_log_likelihood.py:63-85)."""
+        FP = self._fp(object, precomputed_forward_projection, subset_idx)
+        scale = 1 / (FP + pytomography_amd.delta)
+
+        def operator(x):
+            return self.system_matrix.forward(x, subset_idx) * scale
+        return operator
+
+    def compute_gradient_sf(self, object, precomputed_forward_projection=None, subset_idx=None):
+        """x -> -H x * g / (FP + delta)^2  (poisson_log_likelihood.py:87-114)."""
+        g = self._get_projection_subset(self.projections, subset_idx)
+        FP = self._fp(object, precomputed_forward_projection, subset_idx)
+        weight = g / (FP + pytomography_amd.delta) ** 2
+
+        def operator(x):
+            return -self.system_matrix.forward(x, subset_idx) * weight
+        return operator

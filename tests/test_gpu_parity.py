@@ -258,3 +258,32 @@ def test_attenuation_field_matches_suffix_table(cuda, dt):
         e = rel_l2(x, z)
         print(f"{dt} {name}: field vs suffix table rel-L2 {e:.1e}, max |diff| {float((x - z).abs().max()):.1e}")
         assert e <= (1e-6 if dt == torch.float32 else 1e-14)
+
+
+def test_second_order_likelihood_operators(cuda):
+    """compute_gradient_ff / gf / sf (poisson_log_likelihood.py:38-114) against
+    the same compositions of the oracle's forward and reference backward."""
+    from pytomography_amd.likelihoods import PoissonLogLikelihood
+    pa = problem_arrays(24, 12, seed=17, att=True, psf=True)
+    sm = product_system_matrix(pa)
+    sm.set_n_subsets(3)
+    geo = oracle_geometry(pa, torch.float32)
+    gen = torch.Generator().manual_seed(18)
+    f = torch.from_numpy(pa["f"]) + 0.1
+    g = torch.poisson(so.forward(geo, f) * 20, generator=gen)
+    x = torch.rand(f.shape, generator=gen)
+    lk = PoissonLogLikelihood(sm, g.to(cuda))
+    lk._set_n_subsets(3)
+    sub = geo.subsets(3)[1]
+    fp = so.forward(geo, f, sub)
+    hx = so.forward(geo, x, sub)
+    ref = {
+        "ff": -so.backward(geo, hx * g[sub] / (fp ** 2 + so.DELTA), sub),
+        "gf": hx / (fp + so.DELTA),
+        "sf": -hx * g[sub] / (fp + so.DELTA) ** 2,
+    }
+    for name in ("ff", "gf", "sf"):
+        op = getattr(lk, f"compute_gradient_{name}")(f.to(cuda), subset_idx=1)
+        e = rel_l2(op(x.to(cuda)), ref[name])
+        print(f"compute_gradient_{name}: rel-L2 vs oracle {e:.2e}")
+        assert e <= TOL
