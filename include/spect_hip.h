@@ -238,6 +238,14 @@ int spect_rdp_hessian_vjp(int dtype, const void *f, const void *u, void *out, in
                           int32_t ny, int32_t nz, double dx, double dy, double dz, double beta,
                           double gamma, double delta, double scale, void *stream);
 
+/* The prior's derivative of order 2 applied to u: NearestNeighbourPrior.__call__(2)
+ * (priors/nearest_neighbour.py:105-108) for the RDP, i.e.
+ * out = beta*scale * [u * sum_s w phi2_1(f, f_s) + sum_s w phi2_2(f, f_s) u_s]
+ * (replicate-clamped neighbours; FOV scale one). */
+int spect_rdp_hessian_jvp(int dtype, const void *f, const void *u, void *out, int32_t nx,
+                          int32_t ny, int32_t nz, double dx, double dy, double dz, double beta,
+                          double gamma, double delta, double scale, void *stream);
+
 /* Linear preconditioned update of PreconditionedGradientAscentAlgorithm.__call__
  * (preconditioned_gradient_ascent.py:119-122) for BSREM's preconditioner
  * (:461-474):  f = f + (f*(relax/(w*norm_sum + delta)))*((b - norm) - prior);

@@ -412,6 +412,30 @@ def rdp_gradient(f: torch.Tensor, beta: float, gamma: float, dr, beta_scale: flo
     return beta * beta_scale * out * torch.ones(f.shape, dtype=f.dtype)
 
 
+def rdp_hessian_jvp(f: torch.Tensor, u: torch.Tensor, beta: float, gamma: float, dr,
+                    beta_scale: float, delta: float = DELTA) -> torch.Tensor:
+    """NearestNeighbourPrior.__call__(derivative_order=2) (nearest_neighbour.py:105-108)
+    applied to u: diag * u + blur(u), diag = sum_s w phi2_1(f, f_s), blur = sum_s
+    w phi2_2(f, f_s) u_s, with RelativeDifferencePrior.phi2_1 / phi2_2
+    (:179-183); valid points and FOV scale are all ones."""
+    diag = torch.zeros(f.shape, dtype=f.dtype)
+    blur = torch.zeros(f.shape, dtype=f.dtype)
+    dx, dy, dz = dr
+    for i in (-1, 0, 1):
+        for j in (-1, 0, 1):
+            for k in (-1, 0, 1):
+                if i == 0 and j == 0 and k == 0:
+                    continue
+                w = dx / np.sqrt((dx * i) ** 2 + (dy * j) ** 2 + (dz * k) ** 2)
+                fs = nearest_neighbour(f, (i, j, k))
+                us = nearest_neighbour(u, (i, j, k))
+                den = (gamma * torch.abs(f - fs) + f + fs + delta) ** 3
+                diag = diag + (2 * (2 * fs + delta) ** 2) / den * w
+                blur = blur + (-(2 * (2 * f + delta) * (2 * fs + delta)) / den) * us * w
+    c = beta * beta_scale
+    return c * diag * u + c * blur
+
+
 def rdp_value(f: torch.Tensor, beta: float, gamma: float, dr, delta: float = DELTA) -> torch.Tensor:
     """Sum of RelativeDifferencePrior.phi0 (nearest_neighbour.py:173-174) pairs."""
     out = torch.zeros((), dtype=f.dtype)
@@ -494,3 +518,81 @@ def bsrem_unrolled(g: SPECTGeometry, data, n_iters: int, n_subsets: int, beta, g
             f = f + (f * d) * (grad - prior_grad)
             f = torch.where(f <= floor, floor, f)
     return f
+
+
+def gradient_ascent(g: SPECTGeometry, data, n_iters: int, n_subsets: int, method: str,
+                    beta=None, gamma: float = 2.0, additive=None, f0=None,
+                    relaxation=lambda n: 1, floor: float = 0.0, callback=None):
+    """PreconditionedGradientAscentAlgorithm.__call__ (preconditioned_gradient_ascent.py:80-129)
+    with the preconditioner of ``method``: "osem" (:301-329), "osmaposl" (:331-362),
+    "rbiem" (:364-398), "rbimap" (:400-434) or "bsrem" (:436-474), and an RDP prior
+    when ``beta`` is not None.  Written without in-place updates (the clamp is a
+    ``where``) so that autograd can differentiate the result.  ``callback(f, pp, j, k)``
+    sees the iterate before sub-iteration (j, k) and its H_m f + s."""
+    f = torch.ones(g.shape, dtype=g.dtype) if f0 is None else f0
+    if additive is None:
+        additive = torch.zeros(data.shape, dtype=g.dtype)
+    subsets = g.subsets(n_subsets) if n_subsets > 1 else [torch.arange(g.n_angles)]
+    with torch.no_grad():
+        norms = [sensitivity(g, s) for s in subsets]
+        norm_sum = torch.stack(norms).sum(axis=0) if n_subsets > 1 else norms[0]
+    floor_t = torch.tensor(floor, dtype=g.dtype)
+    for j in range(n_iters):
+        for k, sidx in enumerate(subsets):
+            wk = len(sidx) / g.n_angles if n_subsets > 1 else 1
+            pg = 0 if beta is None else rdp_gradient(f, beta, gamma, g.dr, wk)
+            pp = forward(g, f, sidx) + additive[sidx]
+            if callback is not None:
+                callback(f.detach().clone(), pp.detach().clone(), j, k)
+            grad = backward(g, data[sidx] / (pp + DELTA), sidx) - norms[k]
+            nm = norms[k]
+            if method == "osem":
+                c = f * (1 / (nm + DELTA))
+            elif method == "osmaposl":
+                c = f / (nm + pg + DELTA)
+            elif method == "rbiem":
+                rm = torch.max(nm / (norm_sum + DELTA))
+                c = f / (norm_sum * rm + DELTA)
+            elif method == "rbimap":
+                rm = torch.max((nm + pg) / (norm_sum + pg + DELTA))
+                c = f / (norm_sum * rm + pg + DELTA)
+            elif method == "bsrem":
+                c = f * (relaxation(j) / (wk * norm_sum + DELTA))
+            else:
+                raise ValueError(method)
+            f = f + c * (grad - pg)
+            f = torch.where(f <= floor_t, floor_t, f)
+    return f
+
+
+def uncertainty(g: SPECTGeometry, data, objects, fps, mask, n_subsets: int, factor,
+                beta=None, gamma: float = 2.0, additive_variance=None):
+    """LinearPreconditionedGradientAscentAlgorithm.compute_uncertainty
+    (preconditioned_gradient_ascent.py:170-214) with _compute_Q (:216-247) and
+    _compute_B (:249-299), for the last iterate.  ``objects[n]`` is the iterate
+    before sub-iteration n (``objects[-1]`` the result), ``fps[n]`` its H_m f + s,
+    ``factor(m)`` the linear preconditioner factor D of subset m.  Returns
+    (absolute uncertainty, V) with V the (1 or 2, *data.shape) sensitivity."""
+    subsets = g.subsets(n_subsets) if n_subsets > 1 else [torch.arange(g.n_angles)]
+    K = len(objects) - 1
+    q = mask.clone().to(g.dtype)
+    V = torch.zeros((2 if additive_variance is not None else 1, *data.shape), dtype=g.dtype)
+    for n in range(K, 0, -1):
+        m = (n - 1) % n_subsets
+        sidx = subsets[m]
+        cur, fp = objects[n - 1], fps[n - 1]
+        x = q * cur * factor(m)
+        hx = forward(g, x, sidx)
+        V[0, sidx] += hx / (fp + DELTA)
+        if additive_variance is not None:
+            V[1, sidx] += -hx * data[sidx] / (fp + DELTA) ** 2
+        if n > 1:
+            wk = len(sidx) / g.n_angles if n_subsets > 1 else 1
+            out = -backward(g, forward(g, x, sidx) * data[sidx] / (fp ** 2 + DELTA), sidx)
+            if beta is not None:
+                out = out - rdp_hessian_jvp(cur, x, beta, gamma, g.dr, wk)
+            q = out + (objects[n] / (cur + DELTA)) * q
+    abs2 = torch.sum(V[0] * data * V[0])
+    if additive_variance is not None:
+        abs2 = abs2 + torch.sum(additive_variance(V[1]))
+    return float(torch.sqrt(abs2)), V

@@ -27,7 +27,8 @@ EXPORTED = (
     "spect_forward", "spect_backward", "spect_backward_transpose", "spect_osem_subiteration",
     "spect_grad_workspace_bytes", "spect_forward_vjp",
     "spect_batch_create", "spect_batch_run", "spect_batch_destroy",
-    "spect_poisson_ratio", "spect_rdp_gradient", "spect_rdp_hessian_vjp", "spect_bsrem_update",
+    "spect_poisson_ratio", "spect_rdp_gradient", "spect_rdp_hessian_vjp", "spect_rdp_hessian_jvp",
+    "spect_bsrem_update",
 )
 
 
@@ -113,6 +114,9 @@ def _declare(lib):
     lib.spect_poisson_ratio.argtypes = [ci, vp, vp, vp, vp, vp, i64, dbl, vp]
     lib.spect_rdp_gradient.restype = ci
     lib.spect_rdp_gradient.argtypes = [ci, vp, vp, i32, i32, i32, dbl, dbl, dbl, dbl, dbl, dbl, dbl, vp]
+    lib.spect_rdp_hessian_jvp.restype = ci
+    lib.spect_rdp_hessian_jvp.argtypes = [ci, vp, vp, vp, i32, i32, i32, dbl, dbl, dbl, dbl, dbl,
+                                          dbl, dbl, vp]
     lib.spect_rdp_hessian_vjp.restype = ci
     lib.spect_rdp_hessian_vjp.argtypes = [ci, vp, vp, vp, i32, i32, i32, dbl, dbl, dbl, dbl, dbl,
                                           dbl, dbl, vp]

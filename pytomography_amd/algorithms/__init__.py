@@ -1,6 +1,6 @@
 from .preconditioned_gradient_ascent import (PreconditionedGradientAscentAlgorithm,
                                              LinearPreconditionedGradientAscentAlgorithm,
-                                             OSEM, BSREM, MLEM)
+                                             OSEM, BSREM, MLEM, OSMAPOSL, RBIEM, RBIMAP)
 
 __all__ = ["PreconditionedGradientAscentAlgorithm", "LinearPreconditionedGradientAscentAlgorithm",
-           "OSEM", "BSREM", "MLEM"]
+           "OSEM", "BSREM", "MLEM", "OSMAPOSL", "RBIEM", "RBIMAP"]

@@ -80,13 +80,82 @@ class PreconditionedGradientAscentAlgorithm:
 
 
 class LinearPreconditionedGradientAscentAlgorithm(PreconditionedGradientAscentAlgorithm):
-    """C^n(f^n) = D^n f^n (preconditioned_gradient_ascent.py:131-168)."""
+    """C^n(f^n) = D^n f^n (preconditioned_gradient_ascent.py:131-168), and the
+    uncertainty of a masked sum of the reconstruction propagated back through
+    the stored iterates (:170-299)."""
 
     def _linear_preconditioner_factor(self, n_iter: int, n_subset):
         raise NotImplementedError
 
     def _compute_preconditioner(self, object, n_iter, n_subset):
         return object * self._linear_preconditioner_factor(n_iter, n_subset)
+
+    def compute_uncertainty(self, mask: torch.Tensor, data_storage_callback,
+                            subiteration_number: int | None = None, return_pct: bool = False,
+                            include_additive_term: bool = False, post_recon_filter=None):
+        """Absolute (and optionally percent) uncertainty of the sum of the
+        reconstruction over ``mask`` (preconditioned_gradient_ascent.py:170-214)."""
+        if subiteration_number is None:
+            subiteration_number = len(data_storage_callback.objects) - 1
+        final_recon = data_storage_callback.objects[subiteration_number].to(pytomography_amd.device)
+        if post_recon_filter is not None:
+            q = post_recon_filter(mask)
+            final_recon = post_recon_filter(final_recon)
+        else:
+            q = mask.clone()
+        V = 0
+        for n in range(subiteration_number, 0, -1):
+            V += self._compute_B(q, data_storage_callback, n - 1, include_additive_term=include_additive_term)
+            if n > 1:
+                q = self._compute_Q(q, data_storage_callback, n - 1)
+        abs2 = torch.sum(V[0] * self.likelihood.projections * V[0])
+        if include_additive_term:
+            abs2 += torch.sum(self.likelihood.additive_term_variance_estimate(V[1]))
+        uncertainty_abs = torch.sqrt(abs2).item()
+        if not return_pct:
+            return uncertainty_abs
+        return uncertainty_abs, uncertainty_abs / (final_recon * mask).sum().item() * 100
+
+    def _subset(self, n):
+        return None if self.n_subsets == 1 else n % self.n_subsets
+
+    def _compute_Q(self, input: torch.Tensor, data_storage_callback, n: int) -> torch.Tensor:
+        """Q operator of the uncertainty recursion (:216-247)."""
+        subset_idx = self._subset(n)
+        cur = data_storage_callback.objects[n].to(pytomography_amd.device)
+        nxt = data_storage_callback.objects[n + 1].to(pytomography_amd.device)
+        fp = data_storage_callback.projections_predicted[n].to(pytomography_amd.device)
+        grad_ff = self.likelihood.compute_gradient_ff(cur, fp, subset_idx)
+        output = input * cur * self._linear_preconditioner_factor(None, subset_idx)
+        if self.prior is not None:
+            self.prior.set_beta_scale(self.likelihood.system_matrix.get_weighting_subset(subset_idx))
+            self.prior.set_object(cur)
+            output = grad_ff(output) - self.prior(derivative_order=2)(output)
+        else:
+            output = grad_ff(output)
+        output += (nxt / (cur + pytomography_amd.delta)) * input
+        return output
+
+    def _compute_B(self, input: torch.Tensor, data_storage_callback, n: int,
+                   include_additive_term: bool = False) -> torch.Tensor:
+        """B operator of the uncertainty recursion (:249-299): projections of
+        every angle, zero outside the subset."""
+        sm = self.likelihood.system_matrix
+        subset_idx = self._subset(n)
+        if subset_idx is None:
+            rows = torch.arange(sm.proj_meta.shape[0]).to(torch.long).to(pytomography_amd.device)
+        else:
+            rows = sm.subset_indices_array[subset_idx]
+        cur = data_storage_callback.objects[n].to(pytomography_amd.device)
+        fp = data_storage_callback.projections_predicted[n].to(pytomography_amd.device)
+        output = input * cur * self._linear_preconditioner_factor(None, subset_idx)
+        primary = self.likelihood.compute_gradient_gf(cur, fp, subset_idx)(output)
+        shape = (*fp.shape[:-3], *sm.proj_meta.shape[-3:])
+        total = torch.zeros((2 if include_additive_term else 1, *shape)).to(pytomography_amd.device)
+        total[0, ..., rows, :, :] = primary
+        if include_additive_term:
+            total[1, ..., rows, :, :] = self.likelihood.compute_gradient_sf(cur, fp, subset_idx)(output)
+        return total
 
 
 def _fused_ok(alg) -> bool:
@@ -177,3 +246,43 @@ class BSREM(LinearPreconditionedGradientAscentAlgorithm):
             float(self.relaxation_sequence(n_iter)), float(w), float(pytomography_amd.delta),
             float(self.addition_after_iteration), _lib.stream_ptr(f.device)))
         return True
+
+
+class OSMAPOSL(PreconditionedGradientAscentAlgorithm):
+    """One-step-late MAP-EM: C^n = f^n / (H_m^T 1 + grad V(f^n))
+    (preconditioned_gradient_ascent.py:331-362)."""
+
+    def __init__(self, likelihood, object_initial: torch.Tensor | None = None, prior=None):
+        super().__init__(likelihood=likelihood, object_initial=object_initial, prior=prior)
+
+    def _compute_preconditioner(self, object, n_iter, n_subset):
+        return object / (self.likelihood._get_normBP(n_subset) + self.prior_gradient
+                         + pytomography_amd.delta)
+
+
+class RBIEM(LinearPreconditionedGradientAscentAlgorithm):
+    """Rescaled block-iterative EM: C^n = f^n / (sum_m n_m * r_m + delta),
+    r_m = max(n_m / (sum_m n_m + delta)) (preconditioned_gradient_ascent.py:364-398)."""
+
+    def __init__(self, likelihood, object_initial: torch.Tensor | None = None, prior=None):
+        super().__init__(likelihood=likelihood, object_initial=object_initial, prior=prior)
+
+    def _compute_preconditioner(self, object, n_iter, n_subset):
+        norm = self.likelihood._get_normBP(n_subset)
+        total = self.likelihood._get_normBP(n_subset, return_sum=True)
+        rm = torch.max(norm / (total + pytomography_amd.delta))
+        return object / (total * rm + pytomography_amd.delta)
+
+
+class RBIMAP(PreconditionedGradientAscentAlgorithm):
+    """Rescaled block-iterative MAP (preconditioned_gradient_ascent.py:400-434)."""
+
+    def __init__(self, likelihood, object_initial: torch.Tensor | None = None, prior=None):
+        super().__init__(likelihood=likelihood, object_initial=object_initial, prior=prior)
+
+    def _compute_preconditioner(self, object, n_iter, n_subset):
+        norm = self.likelihood._get_normBP(n_subset)
+        total = self.likelihood._get_normBP(n_subset, return_sum=True)
+        v = self.prior_gradient
+        rm = torch.max((norm + v) / (total + v + pytomography_amd.delta))
+        return object / (total * rm + v + pytomography_amd.delta)

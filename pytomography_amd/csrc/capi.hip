@@ -845,6 +845,23 @@ int spect_rdp_hessian_vjp(int dtype, const void *f, const void *u, void *out, in
   return SPECT_OK;
 }
 
+int spect_rdp_hessian_jvp(int dtype, const void *f, const void *u, void *out, int32_t nx,
+                          int32_t ny, int32_t nz, double dx, double dy, double dz, double beta,
+                          double gamma, double delta, double scale, void *stream) {
+  if (!f || !u || !out || nx <= 0 || ny <= 0 || nz <= 0) return fail(SPECT_E_ARG, "bad prior arguments");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const double bs = beta * scale;
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_rdp_jvp<float>((const float *)f, (const float *)u, (float *)out, nx, ny, nz, dx,
+                                  dy, dz, (float)bs, (float)gamma, (float)delta, st));
+  else if (dtype == SPECT_F64)
+    HIP_TRY(launch_rdp_jvp<double>((const double *)f, (const double *)u, (double *)out, nx, ny, nz,
+                                   dx, dy, dz, bs, gamma, delta, st));
+  else
+    return fail(SPECT_E_ARG, "bad dtype");
+  return SPECT_OK;
+}
+
 int spect_bsrem_update(int dtype, void *f, const void *b, const void *norm, const void *norm_sum,
                        const void *prior, int64_t count, double relax, double weight, double delta,
                        double floor_value, void *stream) {

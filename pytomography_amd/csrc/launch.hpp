@@ -159,6 +159,9 @@ template <typename R>
 hipError_t launch_rdp_hvp(const R *f, const R *u, R *out, int nx, int ny, int nz, double dx,
                           double dy, double dz, R c, R gamma, R delta, hipStream_t st);
 template <typename R>
+hipError_t launch_rdp_jvp(const R *f, const R *u, R *out, int nx, int ny, int nz, double dx,
+                          double dy, double dz, R c, R gamma, R delta, hipStream_t st);
+template <typename R>
 hipError_t launch_bsrem_update(R *f, const R *b, const R *nrm, const R *nsum, const R *prior,
                                int64_t count, R relax, R weight, R delta, R floor_value,
                                hipStream_t st);

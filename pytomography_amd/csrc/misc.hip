@@ -121,6 +121,44 @@ __global__ __launch_bounds__(256) void rdp_hvp_kernel(const R *__restrict__ f,
   out[idx] = c * acc;
 }
 
+// The prior's derivative of order 2 applied to u, as NearestNeighbourPrior.__call__(2)
+// (priors/nearest_neighbour.py:105-108): out = c * [u_r sum_s w_s phi2_1(f_r, f_s)]
+// + c * [sum_s w_s phi2_2(f_r, f_s) u_s], s over the replicate-clamped neighbours
+// (the Jacobian of the prior gradient times u; rdp_hvp_kernel is its transpose).
+template <typename R>
+__global__ __launch_bounds__(256) void rdp_jvp_kernel(const R *__restrict__ f,
+                                                      const R *__restrict__ u,
+                                                      R *__restrict__ out, int nx, int ny, int nz,
+                                                      double dx, double dy, double dz, R c,
+                                                      R gamma, R delta) {
+#pragma clang fp contract(off)
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t count = (int64_t)nx * ny * nz;
+  if (idx >= count) return;
+  const int z = idx % nz;
+  const int y = (idx / nz) % ny;
+  const int x = idx / ((int64_t)nz * ny);
+  const R fr = f[idx];
+  R diag = R(0), blur = R(0);
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j)
+      for (int k = -1; k <= 1; ++k) {
+        if (i == 0 && j == 0 && k == 0) continue;
+        const double w = dx / sqrt((dx * i) * (dx * i) + (dy * j) * (dy * j) + (dz * k) * (dz * k));
+        const int64_t s = ((int64_t)min(max(x + i, 0), nx - 1) * ny + min(max(y + j, 0), ny - 1)) * nz +
+                          min(max(z + k, 0), nz - 1);
+        const R fs = f[s];
+        const R den = gamma * fabs(fr - fs) + fr + fs + delta;
+        const R den3 = den * den * den;
+        const R a = R(2) * fs + delta;
+        const R p21 = (R(2) * (a * a)) / den3;
+        const R p22 = -((R(2) * (R(2) * fr + delta)) * (R(2) * fs + delta)) / den3;
+        diag = diag + p21 * R(w);
+        blur = blur + (p22 * u[s]) * R(w);
+      }
+  out[idx] = (c * diag) * u[idx] + c * blur;
+}
+
 // PreconditionedGradientAscentAlgorithm.__call__ (preconditioned_gradient_ascent.py:119-122)
 // with BSREM's preconditioner (:461-474).
 template <typename R>
@@ -172,6 +210,15 @@ hipError_t launch_rdp_hvp(const R *f, const R *u, R *out, int nx, int ny, int nz
 }
 
 template <typename R>
+hipError_t launch_rdp_jvp(const R *f, const R *u, R *out, int nx, int ny, int nz, double dx,
+                          double dy, double dz, R c, R gamma, R delta, hipStream_t st) {
+  const int64_t count = (int64_t)nx * ny * nz;
+  rdp_jvp_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(f, u, out, nx, ny, nz, dx, dy,
+                                                                     dz, c, gamma, delta);
+  return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_bsrem_update(R *f, const R *b, const R *nrm, const R *nsum, const R *prior,
                                int64_t count, R relax, R weight, R delta, R floor_value,
                                hipStream_t st) {
@@ -186,6 +233,8 @@ hipError_t launch_bsrem_update(R *f, const R *b, const R *nrm, const R *nsum, co
   template hipError_t launch_rdp<R>(const R *, R *, int, int, int, double, double, double, R,  \
                                     R, R, hipStream_t);                                        \
   template hipError_t launch_rdp_hvp<R>(const R *, const R *, R *, int, int, int, double,      \
+                                        double, double, R, R, R, hipStream_t);                 \
+  template hipError_t launch_rdp_jvp<R>(const R *, const R *, R *, int, int, int, double,      \
                                         double, double, R, R, R, hipStream_t);                 \
   template hipError_t launch_bsrem_update<R>(R *, const R *, const R *, const R *, const R *,  \
                                              int64_t, R, R, R, R, hipStream_t);

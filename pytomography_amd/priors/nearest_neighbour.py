@@ -95,6 +95,28 @@ class RelativeDifferencePrior(NearestNeighbourPrior):
             float(self.gamma), float(self.delta), float(beta_scale), _lib.stream_ptr(f.device)))
         return out
 
+    def hessian_jvp(self, object: torch.Tensor, u: torch.Tensor, beta_scale: float,
+                    beta: float | None = None) -> torch.Tensor:
+        """The derivative of order 2 applied to u, as NearestNeighbourPrior.__call__(2)
+        (nearest_neighbour.py:105-108): beta*beta_scale*[u sum_s w phi2_1 + sum_s w phi2_2 u_s]
+        times the FOV scale."""
+        dt = torch.float64 if object.dtype == torch.float64 else torch.float32
+        f = _lib.require_device_tensor(object, "object", dt)
+        uu = _lib.require_device_tensor(u, "u", dt)
+        if uu.shape != f.shape:
+            raise ValueError(f"u shape {tuple(uu.shape)} != object shape {tuple(f.shape)}")
+        out = torch.empty_like(f)
+        m = self.object_meta
+        nx, ny, nz = f.shape
+        _lib.check(_lib.lib().spect_rdp_hessian_jvp(
+            _lib.dtype_code(dt), f.data_ptr(), uu.data_ptr(), out.data_ptr(), nx, ny, nz,
+            float(m.dx), float(m.dy), float(m.dz), float(self.beta if beta is None else beta),
+            float(self.gamma), float(self.delta), float(beta_scale), _lib.stream_ptr(f.device)))
+        fov = getattr(self, "FOV_scale", None)
+        if fov is not None and not bool(torch.all(fov == 1)):
+            out = out * fov
+        return out
+
     def value(self, object: torch.Tensor) -> float:
         """sum_r sum_s w_rs phi0(f_r, f_s) * beta (derivative order 0)."""
         f = object
@@ -117,4 +139,7 @@ class RelativeDifferencePrior(NearestNeighbourPrior):
             return self.value(self.object)
         if derivative_order == 1:
             return self.gradient(self.object, self.beta_scale_factor)
-        raise NotImplementedError("only derivative orders 0 and 1 are on the accelerated path")
+        if derivative_order == 2:
+            f, scale = self.object, self.beta_scale_factor
+            return lambda input: self.hessian_jvp(f, input, scale)
+        raise NotImplementedError("Prior not implemented for derivative order >2")

@@ -300,3 +300,104 @@ def test_unrolled_bsrem_gradient_matches_finite_differences():
         fd_b = (loss(data, x0, 0.1 + h) - loss(data, x0, 0.1 - h)) / (2 * h)
     for fd, an in ((fd_d, (dv.grad * dd).sum()), (fd_f, (fv.grad * df).sum()), (fd_b, bv.grad)):
         assert abs(float(fd) - float(an)) <= 1e-5 * abs(float(fd)), (float(fd), float(an))
+
+
+# --------------------------------------------- prior derivative of order 2
+@pytest.mark.parametrize("gamma", [0.0, 2.0])
+def test_rdp_hessian_jvp_is_jacobian_of_gradient(gamma):
+    """phi2_1 / phi2_2 are the exact partials of phi1, and the replicated border
+    neighbours move with the voxel, so the order-2 operator applied to u is the
+    Jacobian of the prior gradient times u (autograd forward-mode), everywhere."""
+    f = _rand((6, 5, 7), seed=21) + 0.2
+    f[1:3, 1:3, 2:4] = 0.0
+    u = _rand((6, 5, 7), seed=22) - 0.5
+    dr = (0.48, 0.5, 0.52)
+    _, jvp = torch.autograd.functional.jvp(lambda x: so.rdp_gradient(x, 0.2, gamma, dr, 0.5), f, u)
+    out = so.rdp_hessian_jvp(f, u, 0.2, gamma, dr, 0.5)
+    assert torch.allclose(out, jvp, rtol=1e-11, atol=1e-11 * float(jvp.abs().max()))
+
+
+# ------------------------------------------ preconditioned gradient ascent
+def _small_problem(n_angles=6, quarter=False):
+    from tests.helpers import oracle_geometry, problem_arrays
+    pa = problem_arrays(8, n_angles, seed=23, nz=4, att=True, psf=True)
+    if quarter:
+        pa["angles"] = np.array([0.0, 90.0, 180.0, 270.0])
+    g = oracle_geometry(pa, torch.float64)
+    f_true = torch.from_numpy(pa["f"]).double() + 0.05
+    expected = so.forward(g, f_true) * 40
+    s = torch.full_like(expected, 0.05 * float(expected.mean()))
+    data = torch.poisson(expected + s, generator=torch.Generator().manual_seed(24)) + 1
+    return g, data, s
+
+
+def test_gradient_ascent_reduces_to_pinned_loops():
+    g, data, s = _small_problem()
+    osem = so.osem(g, data, 2, 3, additive=s)
+    bsrem = so.bsrem(g, data, 2, 3, 0.1, 2.0, additive=s)
+    run = lambda m, **kw: so.gradient_ascent(g, data, 2, 3, m, additive=s, **kw)
+    assert torch.allclose(run("osem"), osem, rtol=1e-13, atol=0)
+    assert torch.allclose(run("bsrem", beta=0.1, floor=1e-4), bsrem, rtol=1e-13, atol=0)
+    # OSMAPOSL without a prior is OSEM; RBI-MAP without a prior is RBI-EM.
+    assert torch.allclose(run("osmaposl"), osem, rtol=1e-13, atol=0)
+    assert torch.allclose(run("rbimap"), run("rbiem"), rtol=1e-13, atol=0)
+    # RBI-EM with one block: r = max(n / (n + delta)) = 1 - O(delta), i.e. MLEM.
+    one = so.gradient_ascent(g, data, 2, 1, "rbiem", additive=s)
+    assert torch.allclose(one, so.osem(g, data, 2, 1, additive=s), rtol=1e-9, atol=0)
+
+
+def test_one_step_late_and_rbi_closed_forms():
+    """One sub-iteration in the textbook forms: OSL-EM (Green 1990)
+    f' = f B(g/p) / (n_m + grad V); RBI-EM (Byrne 1996)
+    f' = f + f (B(g/p) - n_m) / (r_m N), r_m = max n_m / N."""
+    g, data, s = _small_problem()
+    sidx = g.subsets(3)[0]
+    f0 = torch.ones(g.shape, dtype=torch.float64)
+    norms = [so.sensitivity(g, x) for x in g.subsets(3)]
+    N = torch.stack(norms).sum(0)
+    ratio_bp = so.backward(g, data[sidx] / (so.forward(g, f0, sidx) + s[sidx] + so.DELTA), sidx)
+    pg = so.rdp_gradient(f0 + 0, 0.3, 2.0, g.dr, len(sidx) / g.n_angles)
+    got = []
+    so.gradient_ascent(g, data, 1, 3, "osmaposl", beta=0.3, additive=s,
+                       callback=lambda f, pp, j, k: got.append(f))
+    assert len(got) == 3
+    # got[1] is the iterate after sub-iteration 0
+    assert torch.allclose(got[1], f0 * ratio_bp / (norms[0] + pg), rtol=1e-10, atol=0)
+    got = []
+    so.gradient_ascent(g, data, 1, 3, "rbiem", additive=s, callback=lambda f, pp, j, k: got.append(f))
+    rm = float((norms[0] / N).max())
+    assert torch.allclose(got[1], f0 + f0 * (ratio_bp - norms[0]) / (rm * N), rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("method", ["osem", "bsrem"])
+def test_uncertainty_recursion_is_autograd_sensitivity(method):
+    """At quarter-turn angles the reference back projection is the exact
+    transpose of the forward projection, so the Q/B recursion must equal the
+    reverse-mode derivative of sum(mask * f_K) with respect to the projections
+    (V[0]) and to the additive term (V[1])."""
+    g, data, s = _small_problem(quarter=True)
+    assert torch.allclose(so.backward(g, data), so.exact_adjoint(g, data), rtol=1e-12, atol=1e-12)
+    n_sub, iters = 2, 3
+    beta = 0.3 if method == "bsrem" else None
+    floor = 1e-4 if method == "bsrem" else 0.0
+    objects, fps = [], []
+    dv, sv = data.clone().requires_grad_(True), s.clone().requires_grad_(True)
+    fK = so.gradient_ascent(g, dv, iters, n_sub, method, beta=beta, additive=sv, floor=floor,
+                            callback=lambda f, pp, j, k: (objects.append(f), fps.append(pp)))
+    assert float(fK.min()) > floor
+    objects.append(fK.detach().clone())
+    mask = torch.zeros(g.shape, dtype=torch.float64)
+    mask[2:6, 3:6, 1:3] = 1
+    gd, gs = torch.autograd.grad((mask * fK).sum(), (dv, sv))
+    norms = [so.sensitivity(g, x) for x in g.subsets(n_sub)]
+    N = torch.stack(norms).sum(0)
+    if method == "osem":
+        factor = lambda m: 1 / (norms[m] + so.DELTA)
+    else:
+        factor = lambda m: 1 / (0.5 * N + so.DELTA)
+    unc, V = so.uncertainty(g, data, objects, fps, mask, n_sub, factor, beta=beta,
+                            additive_variance=lambda v: v ** 2)
+    assert torch.allclose(V[0], gd, rtol=1e-8, atol=1e-10 * float(gd.abs().max()))
+    assert torch.allclose(V[1], gs, rtol=1e-8, atol=1e-10 * float(gs.abs().max()))
+    expected = math.sqrt(float((gd ** 2 * data).sum() + (gs ** 2).sum()))
+    assert abs(unc - expected) <= 1e-8 * expected
