@@ -47,6 +47,9 @@ enum { SPECT_F32 = 0, SPECT_F64 = 1 };
  * (mu must still be non-NULL; it is not read). */
 enum { SPECT_ATT_FIELD = 0x100 };
 
+/* Padding modes of spect_smooth_axis (torch Conv1d padding_mode). */
+enum { SPECT_PAD_CIRCULAR = 0, SPECT_PAD_REPLICATE = 1, SPECT_PAD_ZEROS = 2 };
+
 /* Rotation used by the back projection. */
 enum {
   SPECT_BP_REFERENCE = 0, /* gather at 270-beta: spect_system_matrix.py:196 */
@@ -254,6 +257,22 @@ int spect_bsrem_update(int dtype, void *f, const void *b, const void *norm,
                        const void *norm_sum, const void *prior, int64_t count,
                        double relax, double weight, double delta, double floor_value,
                        void *stream);
+
+/* Energy-window scatter estimate, compute_EW_scatter (utils/scatter.py:52-99)
+ * before smoothing: estimate = (lower/wl*kl + upper/wu*ku)*wp, and, when
+ * variance_diag is not NULL, the diagonal of its variance estimate
+ * (wp/wl*kl)^2*lower + (wp/wu*ku)^2*upper (:86).  upper may be NULL (DEW:
+ * its terms are 0).  Element-wise over count values. */
+int spect_ew_scatter(int dtype, const void *lower, const void *upper, void *estimate,
+                     void *variance_diag, int64_t count, double width_lower, double width_upper,
+                     double width_peak, double weighting_lower, double weighting_upper,
+                     void *stream);
+
+/* One axis of get_smoothed_scatter (utils/scatter.py:10-49): the 'same'
+ * correlation of an [n0, n1, n2] array along `axis` with K (odd) device-resident
+ * taps, padding mode SPECT_PAD_*.  in and out must not overlap. */
+int spect_smooth_axis(int dtype, const void *in, void *out, int32_t n0, int32_t n1, int32_t n2,
+                      int32_t axis, const void *taps, int32_t K, int32_t mode, void *stream);
 
 #ifdef __cplusplus
 }

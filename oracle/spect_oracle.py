@@ -596,3 +596,54 @@ def uncertainty(g: SPECTGeometry, data, objects, fps, mask, n_subsets: int, fact
     if additive_variance is not None:
         abs2 = abs2 + torch.sum(additive_variance(V[1]))
     return float(torch.sqrt(abs2)), V
+
+
+# --------------------------------------------------------------------------
+# energy-window scatter estimate -- utils/scatter.py:10-99
+# --------------------------------------------------------------------------
+def gaussian_kernel_1d(sigma: float, kernel_size: int, dtype=torch.float32) -> torch.Tensor:
+    """get_1d_gaussian_kernel's weights (utils/misc.py:78-94)."""
+    x = torch.arange(-int(kernel_size // 2), int(kernel_size // 2) + 1)
+    k = torch.exp((-x ** 2 / (2 * sigma ** 2)).to(dtype))
+    return k / k.sum()
+
+
+def smoothed_scatter(scatter: torch.Tensor, dtheta: float, dr: float, dz: float,
+                     sigma_theta: float = 0, sigma_r: float = 0, sigma_z: float = 0,
+                     N_sigmas: int = 3, delta: float = DELTA) -> torch.Tensor:
+    """get_smoothed_scatter (utils/scatter.py:10-50): per axis a Conv1d
+    (padding 'same', circular in theta, replicate in r and z)."""
+    dS = [dtheta, dr, dz]
+    sigmas = [sigma_theta, sigma_r, sigma_z]
+    ksize = [2 * int(np.ceil(N_sigmas * s / d)) + 1 for s, d in zip(sigmas, dS)]
+    modes = ["circular", "replicate", "replicate"]
+    for i in range(3):
+        if sigmas[i] > delta:
+            k = gaussian_kernel_1d(sigmas[i] / dS[i], ksize[i], scatter.dtype)
+            layer = torch.nn.Conv1d(1, 1, ksize[i], padding="same", padding_mode=modes[i],
+                                    bias=False).to(scatter.dtype)
+            layer.weight.data = k.reshape(1, 1, -1)
+            with torch.no_grad():
+                x = scatter.swapaxes(i, 2)
+                x = layer(x.flatten(end_dim=-2).unsqueeze(1)).reshape(x.shape)
+                scatter = x.swapaxes(i, 2)
+    return scatter
+
+
+def ew_scatter(lower, upper, wl, wu, wp, kl=0.5, ku=0.5, dtheta=None, dr=None, dz=None,
+               sigma_theta=0, sigma_r=0, sigma_z=0, N_sigmas=3):
+    """compute_EW_scatter (utils/scatter.py:52-99); returns (estimate,
+    variance diagonal, variance operator)."""
+    upper = 0 if upper is None else upper
+    wu = 1 if wu is None else wu
+    est = (lower / wl * kl + upper / wu * ku) * wp
+    smooth = (sigma_r > 0) + (sigma_theta > 0) + (sigma_z > 0)
+    sm = lambda x: smoothed_scatter(x, dtheta, dr, dz, sigma_theta, sigma_r, sigma_z, N_sigmas)
+    if smooth:
+        est = sm(est)
+    diag = (wp / wl * kl) ** 2 * lower + (wp / wu * ku) ** 2 * upper
+    if smooth:
+        op = lambda x: sm(x) * diag * sm(x)
+    else:
+        op = lambda x: x * diag * x
+    return est, diag, op

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libspect_hip.so")
 SPECT_F32, SPECT_F64 = 0, 1
 ATT_FIELD = 0x100  # OR-ed into dtype: att_table is an attenuation field
 BP_REFERENCE, BP_ADJOINT = 0, 1
+PAD_CIRCULAR, PAD_REPLICATE, PAD_ZEROS = 0, 1, 2
 OUT_STORE, OUT_ADD, OUT_OSEM = 0, 1, 2
 
 EXPORTED = (
@@ -28,7 +29,7 @@ EXPORTED = (
     "spect_grad_workspace_bytes", "spect_forward_vjp",
     "spect_batch_create", "spect_batch_run", "spect_batch_destroy",
     "spect_poisson_ratio", "spect_rdp_gradient", "spect_rdp_hessian_vjp", "spect_rdp_hessian_jvp",
-    "spect_bsrem_update",
+    "spect_bsrem_update", "spect_ew_scatter", "spect_smooth_axis",
 )
 
 
@@ -120,6 +121,10 @@ def _declare(lib):
     lib.spect_rdp_hessian_vjp.restype = ci
     lib.spect_rdp_hessian_vjp.argtypes = [ci, vp, vp, vp, i32, i32, i32, dbl, dbl, dbl, dbl, dbl,
                                           dbl, dbl, vp]
+    lib.spect_ew_scatter.restype = ci
+    lib.spect_ew_scatter.argtypes = [ci, vp, vp, vp, vp, i64, dbl, dbl, dbl, dbl, dbl, vp]
+    lib.spect_smooth_axis.restype = ci
+    lib.spect_smooth_axis.argtypes = [ci, vp, vp, i32, i32, i32, i32, vp, i32, i32, vp]
     lib.spect_bsrem_update.restype = ci
     lib.spect_bsrem_update.argtypes = [ci, vp, vp, vp, vp, vp, i64, dbl, dbl, dbl, dbl, vp]
 

@@ -882,4 +882,56 @@ int spect_bsrem_update(int dtype, void *f, const void *b, const void *norm, cons
   return SPECT_OK;
 }
 
+int spect_ew_scatter(int dtype, const void *lower, const void *upper, void *estimate,
+                     void *variance_diag, int64_t count, double width_lower, double width_upper,
+                     double width_peak, double weighting_lower, double weighting_upper,
+                     void *stream) {
+  if (!lower || !estimate || count < 0) return fail(SPECT_E_ARG, "bad scatter arguments");
+  if (width_lower == 0.0 || (upper && width_upper == 0.0))
+    return fail(SPECT_E_ARG, "energy window widths must be non-zero");
+  if (count == 0) return SPECT_OK;
+  if (count > (int64_t)256 * 0x7fffffff) return fail(SPECT_E_ARG, "array too large");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  // the variance coefficients are Python float (double) expressions (scatter.py:86)
+  const double a1 = width_peak / width_lower * weighting_lower;
+  const double a2 = upper ? width_peak / width_upper * weighting_upper : 0.0;
+  const double c1 = a1 * a1, c2 = a2 * a2;
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_ew_scatter<float>((const float *)lower, (const float *)upper, (float *)estimate,
+                                     (float *)variance_diag, count, (float)width_lower,
+                                     (float)width_upper, (float)width_peak, (float)weighting_lower,
+                                     (float)weighting_upper, (float)c1, (float)c2, st));
+  else if (dtype == SPECT_F64)
+    HIP_TRY(launch_ew_scatter<double>((const double *)lower, (const double *)upper,
+                                      (double *)estimate, (double *)variance_diag, count,
+                                      width_lower, width_upper, width_peak, weighting_lower,
+                                      weighting_upper, c1, c2, st));
+  else
+    return fail(SPECT_E_ARG, "bad dtype");
+  return SPECT_OK;
+}
+
+int spect_smooth_axis(int dtype, const void *in, void *out, int32_t n0, int32_t n1, int32_t n2,
+                      int32_t axis, const void *taps, int32_t K, int32_t mode, void *stream) {
+  if (!in || !out || !taps || n0 <= 0 || n1 <= 0 || n2 <= 0)
+    return fail(SPECT_E_ARG, "bad smoothing arguments");
+  if (in == out) return fail(SPECT_E_ARG, "smoothing input and output must differ");
+  if (axis < 0 || axis > 2) return fail(SPECT_E_ARG, "axis must be 0, 1 or 2");
+  if (K < 1 || (K % 2) == 0) return fail(SPECT_E_ARG, "kernel size must be odd and positive");
+  if (mode != SPECT_PAD_CIRCULAR && mode != SPECT_PAD_REPLICATE && mode != SPECT_PAD_ZEROS)
+    return fail(SPECT_E_ARG, "bad padding mode");
+  if ((int64_t)n0 * n1 * n2 > (int64_t)256 * 0x7fffffff)
+    return fail(SPECT_E_ARG, "array too large");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_smooth_axis<float>((const float *)in, (float *)out, n0, n1, n2, axis,
+                                      (const float *)taps, K, mode, st));
+  else if (dtype == SPECT_F64)
+    HIP_TRY(launch_smooth_axis<double>((const double *)in, (double *)out, n0, n1, n2, axis,
+                                       (const double *)taps, K, mode, st));
+  else
+    return fail(SPECT_E_ARG, "bad dtype");
+  return SPECT_OK;
+}
+
 }  // extern "C"

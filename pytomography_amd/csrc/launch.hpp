@@ -166,4 +166,12 @@ hipError_t launch_bsrem_update(R *f, const R *b, const R *nrm, const R *nsum, co
                                int64_t count, R relax, R weight, R delta, R floor_value,
                                hipStream_t st);
 
+// Energy-window scatter combination and one-axis smoothing (scatter.hip).
+template <typename R>
+hipError_t launch_ew_scatter(const R *lower, const R *upper, R *est, R *var, int64_t count, R wl,
+                             R wu, R wp, R kl, R ku, R c1, R c2, hipStream_t st);
+template <typename R>
+hipError_t launch_smooth_axis(const R *in, R *out, int n0, int n1, int n2, int axis, const R *taps,
+                              int K, int mode, hipStream_t st);
+
 }  // namespace spect

@@ -401,3 +401,45 @@ def test_uncertainty_recursion_is_autograd_sensitivity(method):
     assert torch.allclose(V[1], gs, rtol=1e-8, atol=1e-10 * float(gs.abs().max()))
     expected = math.sqrt(float((gd ** 2 * data).sum() + (gs ** 2).sum()))
     assert abs(unc - expected) <= 1e-8 * expected
+
+
+# ------------------------------------------------- energy-window scatter
+def test_ew_scatter_closed_form():
+    lo = torch.tensor([[[10.0, 20.0]]], dtype=torch.float64)
+    up = torch.tensor([[[4.0, 0.0]]], dtype=torch.float64)
+    est, diag, op = so.ew_scatter(lo, up, 6.0, 4.0, 20.0, 0.5, 0.5)
+    # TEW trapezoid: (C_l/w_l + C_u/w_u) * w_p / 2
+    assert torch.allclose(est, torch.tensor([[[(10 / 6 + 4 / 4) * 10, (20 / 6) * 10]]], dtype=torch.float64))
+    assert torch.allclose(diag, (20 / 6 * 0.5) ** 2 * lo + (20 / 4 * 0.5) ** 2 * up)
+    x = torch.tensor([[[2.0, -1.0]]], dtype=torch.float64)
+    assert torch.allclose(op(x), x * diag * x)
+    # DEW: no upper window
+    est, diag, _ = so.ew_scatter(lo, None, 6.0, None, 20.0, 1.0, 0.5)
+    assert torch.equal(est, lo / 6.0 * 1.0 * 20.0)
+
+
+@pytest.mark.parametrize("axis,mode", [(0, "wrap"), (1, "nearest"), (2, "nearest")])
+def test_smoothed_scatter_matches_scipy_correlate(axis, mode):
+    from scipy.ndimage import correlate1d
+    x = _rand((12, 9, 7), seed=31)
+    sig = [0.0, 0.0, 0.0]
+    d = [3.0, 0.48, 0.5]
+    sig[axis] = [7.0, 0.9, 0.6][axis]
+    out = so.smoothed_scatter(x, d[0], d[1], d[2], sig[0], sig[1], sig[2])
+    K = 2 * int(np.ceil(3 * sig[axis] / d[axis])) + 1
+    w = so.gaussian_kernel_1d(sig[axis] / d[axis], K, torch.float64).numpy()
+    ref = correlate1d(x.numpy(), w, axis=axis, mode=mode)
+    assert np.abs(out.numpy() - ref).max() < 1e-12
+    # normalised taps: a constant stays constant
+    c = so.smoothed_scatter(torch.full((12, 9, 7), 3.0, dtype=torch.float64), d[0], d[1], d[2], *sig)
+    assert torch.allclose(c, torch.full_like(c, 3.0), rtol=1e-14, atol=0)
+
+
+def test_gaussian_taps_closed_form():
+    # the exponent is an int64 tensor over a Python float: torch's default
+    # (float32) dtype, as in the reference
+    k = so.gaussian_kernel_1d(1.5, 7, torch.float64)
+    x = np.arange(-3, 4)
+    ref = np.exp(-x ** 2 / (2 * 1.5 ** 2))
+    assert np.allclose(k.numpy(), ref / ref.sum(), rtol=2e-7, atol=0)
+    assert torch.equal(k, k.flip(0))
