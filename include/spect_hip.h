@@ -274,6 +274,20 @@ int spect_ew_scatter(int dtype, const void *lower, const void *upper, void *esti
 int spect_smooth_axis(int dtype, const void *in, void *out, int32_t n0, int32_t n1, int32_t n2,
                       int32_t axis, const void *taps, int32_t K, int32_t mode, void *stream);
 
+/* Kernel-EM basis K applied to a coefficient image, KEMTransform.forward /
+ * backward (transforms/shared/kem.py:113-146; K is symmetric so both are this):
+ * out(v) = sum_{i,j,k in [-m,m]} kernel[i][j][k](v) * f(clamp(v + (i,j,k))),
+ * m = (size-1)/2, with kernel laid out [size^3][nx*ny*nz] and neighbours
+ * replicate-clamped (utils/misc.py:40-57).  f and out must not overlap. */
+int spect_kem_apply(int dtype, const void *kernel, const void *f, void *out, int32_t nx,
+                    int32_t ny, int32_t nz, int32_t size, void *stream);
+
+/* DVFMotionTransform forward / backward (transforms/shared/motion.py:54-110):
+ * out(v) = trilinear sample (zeros outside, align_corners) of f at v + dvf(v),
+ * dvf laid out [nx][ny][nz][3].  nx, ny, nz >= 2; f and out must not overlap. */
+int spect_dvf_warp(int dtype, const void *f, const void *dvf, void *out, int32_t nx, int32_t ny,
+                   int32_t nz, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

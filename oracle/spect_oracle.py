@@ -647,3 +647,57 @@ def ew_scatter(lower, upper, wl, wu, wp, kl=0.5, ku=0.5, dtheta=None, dr=None, d
     else:
         op = lambda x: x * diag * x
     return est, diag, op
+
+
+# --------------------------------------------------------------------------
+# wrappers of the projector -- transforms/shared/kem.py, motion.py,
+# projectors/system_matrix.py:117-221
+# --------------------------------------------------------------------------
+def kem_kernel(support_objects, dr, size=5, sigmas=None, distance_sigma=1.0, top_N=None,
+               dtype=torch.float32):
+    """KEMTransform.compute_kernel (kem.py:64-96) with the default Gaussian
+    support and distance kernels; returns [size, size, size, *shape]."""
+    sigmas = [1.0] * len(support_objects) if sigmas is None else sigmas
+    m = (size - 1) // 2
+    idxs = torch.arange(-m, m + 1)
+    shape = support_objects[0].shape
+    K = torch.ones((size, size, size, *shape)).to(dtype)
+    for i in idxs:
+        for j in idxs:
+            for k in idxs:
+                comp = 1
+                for so, sg in zip(support_objects, sigmas):
+                    nb = nearest_neighbour(so, (int(i), int(j), int(k)))
+                    comp *= torch.exp(-(so - nb) ** 2 / (2 * sg ** 2))
+                K[i + m, j + m, k + m] = comp
+    if top_N is not None:
+        flat = K.reshape((size ** 3, *shape))
+        K *= (torch.argsort(torch.argsort(flat, dim=0), dim=0) >= size ** 3 - top_N).reshape(K.shape)
+    xv, yv, zv = torch.meshgrid([dr[0] * idxs, dr[1] * idxs, dr[2] * idxs], indexing="ij")
+    d = torch.sqrt(xv ** 2 + yv ** 2 + zv ** 2)
+    K *= np.exp(-d ** 2 / (2 * distance_sigma ** 2)).unsqueeze(-1).unsqueeze(-1).unsqueeze(-1)
+    K /= K.sum(dim=(0, 1, 2))
+    return K
+
+
+def kem_apply(K: torch.Tensor, f: torch.Tensor) -> torch.Tensor:
+    """KEMTransform.forward (kem.py:113-131): sum_s K[s] * f(clamp(v + s))."""
+    size = K.shape[0]
+    m = (size - 1) // 2
+    out = torch.zeros(f.shape, dtype=f.dtype)
+    for i in range(-m, m + 1):
+        for j in range(-m, m + 1):
+            for k in range(-m, m + 1):
+                out += K[i + m, j + m, k + m] * nearest_neighbour(f, (i, j, k))
+    return out
+
+
+def dvf_warp(f: torch.Tensor, dvf: torch.Tensor) -> torch.Tensor:
+    """DVFMotionTransform._apply_dvf (motion.py:40-78) with vol_ratio 1."""
+    dims = torch.tensor(f.shape)
+    old = torch.stack(torch.meshgrid(torch.arange(f.shape[0]), torch.arange(f.shape[1]),
+                                     torch.arange(f.shape[2]), indexing="ij")).permute((1, 2, 3, 0)).to(f.dtype)
+    new = old + dvf
+    new = 2 / (dims - 1) * new - 1
+    return F.grid_sample(f.unsqueeze(0).unsqueeze(0), new.to(f.dtype).unsqueeze(0).flip(dims=[-1]),
+                         align_corners=True).squeeze() * 1

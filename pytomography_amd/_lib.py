@@ -30,6 +30,7 @@ EXPORTED = (
     "spect_batch_create", "spect_batch_run", "spect_batch_destroy",
     "spect_poisson_ratio", "spect_rdp_gradient", "spect_rdp_hessian_vjp", "spect_rdp_hessian_jvp",
     "spect_bsrem_update", "spect_ew_scatter", "spect_smooth_axis",
+    "spect_kem_apply", "spect_dvf_warp",
 )
 
 
@@ -125,6 +126,10 @@ def _declare(lib):
     lib.spect_ew_scatter.argtypes = [ci, vp, vp, vp, vp, i64, dbl, dbl, dbl, dbl, dbl, vp]
     lib.spect_smooth_axis.restype = ci
     lib.spect_smooth_axis.argtypes = [ci, vp, vp, i32, i32, i32, i32, vp, i32, i32, vp]
+    lib.spect_kem_apply.restype = ci
+    lib.spect_kem_apply.argtypes = [ci, vp, vp, vp, i32, i32, i32, i32, vp]
+    lib.spect_dvf_warp.restype = ci
+    lib.spect_dvf_warp.argtypes = [ci, vp, vp, vp, i32, i32, i32, vp]
     lib.spect_bsrem_update.restype = ci
     lib.spect_bsrem_update.argtypes = [ci, vp, vp, vp, vp, vp, i64, dbl, dbl, dbl, dbl, vp]
 

@@ -934,4 +934,39 @@ int spect_smooth_axis(int dtype, const void *in, void *out, int32_t n0, int32_t 
   return SPECT_OK;
 }
 
+int spect_kem_apply(int dtype, const void *kernel, const void *f, void *out, int32_t nx,
+                    int32_t ny, int32_t nz, int32_t size, void *stream) {
+  if (!kernel || !f || !out || nx <= 0 || ny <= 0 || nz <= 0)
+    return fail(SPECT_E_ARG, "bad KEM arguments");
+  if (size < 1 || (size % 2) == 0) return fail(SPECT_E_ARG, "KEM kernel size must be odd");
+  if (f == out) return fail(SPECT_E_ARG, "KEM input and output must differ");
+  if ((int64_t)nx * ny * nz > (int64_t)256 * 0x7fffffff) return fail(SPECT_E_ARG, "object too large");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const int m = (size - 1) / 2;
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_kem<float>((const float *)kernel, (const float *)f, (float *)out, nx, ny, nz, m, st));
+  else if (dtype == SPECT_F64)
+    HIP_TRY(launch_kem<double>((const double *)kernel, (const double *)f, (double *)out, nx, ny, nz,
+                               m, st));
+  else
+    return fail(SPECT_E_ARG, "bad dtype");
+  return SPECT_OK;
+}
+
+int spect_dvf_warp(int dtype, const void *f, const void *dvf, void *out, int32_t nx, int32_t ny,
+                   int32_t nz, void *stream) {
+  if (!f || !dvf || !out) return fail(SPECT_E_ARG, "bad DVF arguments");
+  if (nx < 2 || ny < 2 || nz < 2) return fail(SPECT_E_ARG, "DVF warp needs every side >= 2");
+  if (f == out) return fail(SPECT_E_ARG, "DVF input and output must differ");
+  if ((int64_t)nx * ny * nz > (int64_t)256 * 0x7fffffff) return fail(SPECT_E_ARG, "object too large");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_dvf<float>((const float *)f, (const float *)dvf, (float *)out, nx, ny, nz, st));
+  else if (dtype == SPECT_F64)
+    HIP_TRY(launch_dvf<double>((const double *)f, (const double *)dvf, (double *)out, nx, ny, nz, st));
+  else
+    return fail(SPECT_E_ARG, "bad dtype");
+  return SPECT_OK;
+}
+
 }  // extern "C"

@@ -174,4 +174,11 @@ template <typename R>
 hipError_t launch_smooth_axis(const R *in, R *out, int n0, int n1, int n2, int axis, const R *taps,
                               int K, int mode, hipStream_t st);
 
+// Kernel-EM basis and DVF warp (shared.hip).
+template <typename R>
+hipError_t launch_kem(const R *K, const R *f, R *out, int nx, int ny, int nz, int m,
+                      hipStream_t st);
+template <typename R>
+hipError_t launch_dvf(const R *f, const R *dvf, R *out, int nx, int ny, int nz, hipStream_t st);
+
 }  // namespace spect

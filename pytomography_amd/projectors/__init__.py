@@ -1,3 +1,3 @@
-from .system_matrix import SystemMatrix
+from .system_matrix import SystemMatrix, ExtendedSystemMatrix
 
-__all__ = ["SystemMatrix"]
+__all__ = ["SystemMatrix", "ExtendedSystemMatrix"]

@@ -13,3 +13,17 @@ def get_distance(Lx: int, r: float, dx: float) -> np.ndarray:
         d = r + ((Lx - 1) / 2 - np.arange(Lx)) * dx
     d[d < 0] = 0
     return d
+
+
+def get_object_nearest_neighbour(object, shifts):
+    """The object shifted by ``shifts`` voxels with replicate-clamped edges:
+    out(v) = object(clamp(v + shifts)) (utils/misc.py:40-57).  Host/torch
+    helper for transform set-up; the per-iteration stencils are HIP kernels."""
+    import torch
+    import torch.nn.functional as F
+    shift_max = int(max(abs(int(s)) for s in shifts))
+    if shift_max == 0:
+        return object
+    nb = F.pad(object.unsqueeze(0), 6 * [shift_max], mode="replicate").squeeze(0)
+    nb = torch.roll(nb, shifts=[-int(s) for s in shifts], dims=(0, 1, 2))
+    return nb[shift_max:-shift_max, shift_max:-shift_max, shift_max:-shift_max]

@@ -443,3 +443,67 @@ def test_gaussian_taps_closed_form():
     ref = np.exp(-x ** 2 / (2 * 1.5 ** 2))
     assert np.allclose(k.numpy(), ref / ref.sum(), rtol=2e-7, atol=0)
     assert torch.equal(k, k.flip(0))
+
+
+# ------------------------------------------------ KEM basis and DVF warp
+def test_kem_apply_matches_numpy_brute_force():
+    f = _rand((4, 5, 3), seed=41)
+    K = torch.rand((3, 3, 3, 4, 5, 3), generator=torch.Generator().manual_seed(42), dtype=torch.float64)
+    out = so.kem_apply(K, f).numpy()
+    fn, Kn = f.numpy(), K.numpy()
+    ref = np.zeros_like(fn)
+    for x in range(4):
+        for y in range(5):
+            for z in range(3):
+                for i in range(-1, 2):
+                    for j in range(-1, 2):
+                        for k in range(-1, 2):
+                            xs, ys, zs = min(max(x + i, 0), 3), min(max(y + j, 0), 4), min(max(z + k, 0), 2)
+                            ref[x, y, z] += Kn[i + 1, j + 1, k + 1, x, y, z] * fn[xs, ys, zs]
+    assert np.abs(out - ref).max() < 1e-13
+
+
+def test_kem_kernel_normalised_and_uniform_support_is_distance_only():
+    sup = torch.full((6, 6, 6), 2.0, dtype=torch.float64)
+    K = so.kem_kernel([sup], (0.5, 0.5, 0.5), size=3, dtype=torch.float64)
+    assert torch.allclose(K.sum(dim=(0, 1, 2)), torch.ones(6, 6, 6, dtype=torch.float64))
+    # a constant support object leaves only the Gaussian distance weights
+    x = np.arange(-1, 2) * 0.5
+    d2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
+    w = np.exp(-d2 / 2)
+    assert np.allclose(K[..., 2, 3, 1].numpy(), w / w.sum(), rtol=1e-6, atol=0)
+    # top_N keeps exactly N taps per voxel
+    sup2 = _rand((6, 6, 6), seed=43)
+    K2 = so.kem_kernel([sup2], (0.5, 0.5, 0.5), size=3, top_N=5, dtype=torch.float64)
+    assert torch.all((K2 > 0).sum(dim=(0, 1, 2)) == 5)
+
+
+@pytest.fixture
+def default_float64():
+    """The reference's float64 mode sets torch's default dtype (__init__.py:24-32),
+    which the DVF normalisation 2/(dims-1) is computed in."""
+    old = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    yield
+    torch.set_default_dtype(old)
+
+
+def test_dvf_warp_closed_forms(default_float64):
+    f = _rand((6, 7, 5), seed=44)
+    zero = torch.zeros((6, 7, 5, 3), dtype=torch.float64)
+    assert torch.allclose(so.dvf_warp(f, zero), f, rtol=1e-12, atol=1e-12)
+    # integer displacement: a shift with zeros entering
+    sh = zero.clone()
+    sh[..., 0] = 1.0
+    sh[..., 2] = -2.0
+    out = so.dvf_warp(f, sh)
+    ref = torch.zeros_like(f)
+    ref[:-1, :, 2:] = f[1:, :, :-2]
+    assert torch.allclose(out, ref, rtol=1e-12, atol=1e-12)
+    # fractional displacement inside: trilinear interpolation of a linear ramp is exact
+    ramp = torch.from_numpy(np.fromfunction(lambda x, y, z: 0.3 * x - 0.7 * y + 1.1 * z + 2, (6, 7, 5)))
+    fr = zero.clone()
+    fr[..., 0], fr[..., 1], fr[..., 2] = 0.25, 0.4, -0.35
+    out = so.dvf_warp(ramp, fr)
+    exp = ramp + 0.3 * 0.25 - 0.7 * 0.4 + 1.1 * -0.35
+    assert torch.allclose(out[:5, :6, 1:], exp[:5, :6, 1:], rtol=1e-12, atol=1e-12)
