@@ -2,8 +2,31 @@
 #include "../../include/spect_hip.h"
 #include "launch.hpp"
 
+#include <cmath>
+
 namespace spect {
 namespace {
+
+// EuclideanNeighbourWeight (priors/nearest_neighbour.py:207-220) of the 26
+// neighbours, dx / |(i dx, j dy, k dz)| evaluated in double on the host and
+// rounded to R once; index (i+1)*9 + (j+1)*3 + (k+1), centre unused.  Passed
+// by value, so the unrolled stencils read them from the kernel arguments.
+template <typename R> struct RdpWeights {
+  R w[27];
+};
+
+template <typename R>
+RdpWeights<R> rdp_weights(double dx, double dy, double dz) {
+  RdpWeights<R> W{};
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j)
+      for (int k = -1; k <= 1; ++k) {
+        if (i == 0 && j == 0 && k == 0) continue;
+        const double w = dx / std::sqrt((dx * i) * (dx * i) + (dy * j) * (dy * j) + (dz * k) * (dz * k));
+        W.w[(i + 1) * 9 + (j + 1) * 3 + (k + 1)] = R(w);
+      }
+  return W;
+}
 
 // likelihoods/poisson_log_likelihood.py:34-36
 template <typename R>
@@ -24,8 +47,8 @@ __global__ __launch_bounds__(256) void ratio_kernel(const R *__restrict__ g,
 // EuclideanNeighbourWeight (:207-220), in the reference's loop order.
 template <typename R>
 __global__ __launch_bounds__(256) void rdp_kernel(const R *__restrict__ f, R *__restrict__ out,
-                                                  int nx, int ny, int nz, double dx, double dy,
-                                                  double dz, R beta_scale, R gamma, R delta) {
+                                                  int nx, int ny, int nz, RdpWeights<R> W,
+                                                  R beta_scale, R gamma, R delta) {
 #pragma clang fp contract(off)
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t count = (int64_t)nx * ny * nz;
@@ -35,11 +58,14 @@ __global__ __launch_bounds__(256) void rdp_kernel(const R *__restrict__ f, R *__
   const int x = idx / ((int64_t)nz * ny);
   const R fr = f[idx];
   R acc = R(0);
+#pragma unroll
   for (int i = -1; i <= 1; ++i)
+#pragma unroll
     for (int j = -1; j <= 1; ++j)
+#pragma unroll
       for (int k = -1; k <= 1; ++k) {
         if (i == 0 && j == 0 && k == 0) continue;
-        const double w = dx / sqrt((dx * i) * (dx * i) + (dy * j) * (dy * j) + (dz * k) * (dz * k));
+        const R w = W.w[(i + 1) * 9 + (j + 1) * 3 + (k + 1)];
         const int xs = min(max(x + i, 0), nx - 1);
         const int ys = min(max(y + j, 0), ny - 1);
         const int zs = min(max(z + k, 0), nz - 1);
@@ -48,7 +74,7 @@ __global__ __launch_bounds__(256) void rdp_kernel(const R *__restrict__ f, R *__
         const R ad = fabs(d);
         const R num = d * (gamma * ad + R(3) * fs + fr + R(2) * delta);
         const R den = fr + fs + gamma * ad + delta;
-        acc = acc + (num / (den * den)) * R(w);
+        acc = acc + (num / (den * den)) * w;
       }
   out[idx] = beta_scale * acc;
 }
@@ -64,8 +90,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void rdp_hvp_kernel(const R *__restrict__ f,
                                                       const R *__restrict__ u,
                                                       R *__restrict__ out, int nx, int ny, int nz,
-                                                      double dx, double dy, double dz, R c,
-                                                      R gamma, R delta) {
+                                                      RdpWeights<R> W, R c, R gamma, R delta) {
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t count = (int64_t)nx * ny * nz;
   if (idx >= count) return;
@@ -85,11 +110,14 @@ __global__ __launch_bounds__(256) void rdp_hvp_kernel(const R *__restrict__ f,
     return -(R(2) * (R(2) * fr + delta) * (R(2) * fs + delta)) / (den * den * den);
   };
   R acc = R(0);
+#pragma unroll
   for (int i = -1; i <= 1; ++i)
+#pragma unroll
     for (int j = -1; j <= 1; ++j)
+#pragma unroll
       for (int k = -1; k <= 1; ++k) {
         if (i == 0 && j == 0 && k == 0) continue;
-        const double w = dx / sqrt((dx * i) * (dx * i) + (dy * j) * (dy * j) + (dz * k) * (dz * k));
+        const R w = W.w[(i + 1) * 9 + (j + 1) * 3 + (k + 1)];
         const int d3[3] = {i, j, k};
         // own term: neighbour of q
         const int xs = min(max(x + i, 0), nx - 1), ys = min(max(y + j, 0), ny - 1),
@@ -116,7 +144,7 @@ __global__ __launch_bounds__(256) void rdp_hvp_kernel(const R *__restrict__ f,
               const int64_t r = ((int64_t)cand[0][a] * ny + cand[1][b]) * nz + cand[2][e];
               term += u[r] * phi2_2(f[r], fq);
             }
-        acc += term * R(w);
+        acc += term * w;
       }
   out[idx] = c * acc;
 }
@@ -129,8 +157,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void rdp_jvp_kernel(const R *__restrict__ f,
                                                       const R *__restrict__ u,
                                                       R *__restrict__ out, int nx, int ny, int nz,
-                                                      double dx, double dy, double dz, R c,
-                                                      R gamma, R delta) {
+                                                      RdpWeights<R> W, R c, R gamma, R delta) {
 #pragma clang fp contract(off)
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t count = (int64_t)nx * ny * nz;
@@ -140,11 +167,14 @@ __global__ __launch_bounds__(256) void rdp_jvp_kernel(const R *__restrict__ f,
   const int x = idx / ((int64_t)nz * ny);
   const R fr = f[idx];
   R diag = R(0), blur = R(0);
+#pragma unroll
   for (int i = -1; i <= 1; ++i)
+#pragma unroll
     for (int j = -1; j <= 1; ++j)
+#pragma unroll
       for (int k = -1; k <= 1; ++k) {
         if (i == 0 && j == 0 && k == 0) continue;
-        const double w = dx / sqrt((dx * i) * (dx * i) + (dy * j) * (dy * j) + (dz * k) * (dz * k));
+        const R w = W.w[(i + 1) * 9 + (j + 1) * 3 + (k + 1)];
         const int64_t s = ((int64_t)min(max(x + i, 0), nx - 1) * ny + min(max(y + j, 0), ny - 1)) * nz +
                           min(max(z + k, 0), nz - 1);
         const R fs = f[s];
@@ -153,8 +183,8 @@ __global__ __launch_bounds__(256) void rdp_jvp_kernel(const R *__restrict__ f,
         const R a = R(2) * fs + delta;
         const R p21 = (R(2) * (a * a)) / den3;
         const R p22 = -((R(2) * (R(2) * fr + delta)) * (R(2) * fs + delta)) / den3;
-        diag = diag + p21 * R(w);
-        blur = blur + (p22 * u[s]) * R(w);
+        diag = diag + p21 * w;
+        blur = blur + (p22 * u[s]) * w;
       }
   out[idx] = (c * diag) * u[idx] + c * blur;
 }
@@ -195,8 +225,8 @@ template <typename R>
 hipError_t launch_rdp(const R *f, R *out, int nx, int ny, int nz, double dx, double dy, double dz,
                       R beta_scale, R gamma, R delta, hipStream_t st) {
   const int64_t count = (int64_t)nx * ny * nz;
-  rdp_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(f, out, nx, ny, nz, dx, dy, dz,
-                                                                 beta_scale, gamma, delta);
+  rdp_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(
+      f, out, nx, ny, nz, rdp_weights<R>(dx, dy, dz), beta_scale, gamma, delta);
   return hipGetLastError();
 }
 
@@ -204,8 +234,8 @@ template <typename R>
 hipError_t launch_rdp_hvp(const R *f, const R *u, R *out, int nx, int ny, int nz, double dx,
                           double dy, double dz, R c, R gamma, R delta, hipStream_t st) {
   const int64_t count = (int64_t)nx * ny * nz;
-  rdp_hvp_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(f, u, out, nx, ny, nz, dx, dy,
-                                                                     dz, c, gamma, delta);
+  rdp_hvp_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(
+      f, u, out, nx, ny, nz, rdp_weights<R>(dx, dy, dz), c, gamma, delta);
   return hipGetLastError();
 }
 
@@ -213,8 +243,8 @@ template <typename R>
 hipError_t launch_rdp_jvp(const R *f, const R *u, R *out, int nx, int ny, int nz, double dx,
                           double dy, double dz, R c, R gamma, R delta, hipStream_t st) {
   const int64_t count = (int64_t)nx * ny * nz;
-  rdp_jvp_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(f, u, out, nx, ny, nz, dx, dy,
-                                                                     dz, c, gamma, delta);
+  rdp_jvp_kernel<R><<<(unsigned)((count + 255) / 256), 256, 0, st>>>(
+      f, u, out, nx, ny, nz, rdp_weights<R>(dx, dy, dz), c, gamma, delta);
   return hipGetLastError();
 }
 

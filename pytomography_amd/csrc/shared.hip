@@ -12,11 +12,15 @@ namespace {
 // kem.py:113-131, with get_object_nearest_neighbour's replicate clamp,
 // utils/misc.py:40-57).  K is [S^3][nx*ny*nz]: each tap plane is read once,
 // coalesced along v; f is re-read through L2.
-template <typename R>
+// MT > 0: the half-width m = MT is a compile-time constant and the tap loop is
+// unrolled, so the size^3 independent tap-plane loads are all in flight at
+// once; MT = 0 is the generic loop.
+template <typename R, int MT>
 __global__ __launch_bounds__(256) void kem_kernel(const R *__restrict__ K, const R *__restrict__ f,
                                                   R *__restrict__ out, int nx, int ny, int nz,
-                                                  int m) {
+                                                  int m_rt) {
 #pragma clang fp contract(off)
+  const int m = MT > 0 ? MT : m_rt;
   const int64_t n = (int64_t)nx * ny * nz;
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (v >= n) return;
@@ -25,11 +29,14 @@ __global__ __launch_bounds__(256) void kem_kernel(const R *__restrict__ K, const
   const int x = v / ((int64_t)nz * ny);
   R acc = R(0);
   int t = 0;
+#pragma unroll
   for (int i = -m; i <= m; ++i) {
     const int xs = min(max(x + i, 0), nx - 1);
+#pragma unroll
     for (int j = -m; j <= m; ++j) {
       const int ys = min(max(y + j, 0), ny - 1);
       const R *row = f + ((int64_t)xs * ny + ys) * nz;
+#pragma unroll
       for (int k = -m; k <= m; ++k, ++t) {
         const int zs = min(max(z + k, 0), nz - 1);
         acc = acc + K[(int64_t)t * n + v] * row[zs];
@@ -102,7 +109,12 @@ template <typename R>
 hipError_t launch_kem(const R *K, const R *f, R *out, int nx, int ny, int nz, int m,
                       hipStream_t st) {
   const int64_t n = (int64_t)nx * ny * nz;
-  kem_kernel<R><<<dim3((unsigned)((n + 255) / 256)), 256, 0, st>>>(K, f, out, nx, ny, nz, m);
+  const dim3 grid((unsigned)((n + 255) / 256));
+  switch (m) {
+    case 1: kem_kernel<R, 1><<<grid, 256, 0, st>>>(K, f, out, nx, ny, nz, m); break;
+    case 2: kem_kernel<R, 2><<<grid, 256, 0, st>>>(K, f, out, nx, ny, nz, m); break;
+    default: kem_kernel<R, 0><<<grid, 256, 0, st>>>(K, f, out, nx, ny, nz, m); break;
+  }
   return hipGetLastError();
 }
 
