@@ -55,6 +55,18 @@ class RelativeDifferencePrior(NearestNeighbourPrior):
             delta = pytomography_amd.delta
         super().__init__(beta, weight=weight, gamma=gamma, delta=delta)
 
+    def _fov_factor(self):
+        """The FOV scale, or None when it is all ones (then the multiply is
+        skipped).  The all-ones test syncs with the device, so its answer is
+        kept until the tensor is replaced or modified."""
+        fov = getattr(self, "FOV_scale", None)
+        if fov is None:
+            return None
+        if getattr(self, "_fov_ref", None) is not fov or self._fov_version != fov._version:
+            self._fov_ref, self._fov_version = fov, fov._version
+            self._fov_is_one = bool(torch.all(fov == 1))
+        return None if self._fov_is_one else fov
+
     def gradient(self, object: torch.Tensor, beta_scale: float,
                  beta: float | None = None) -> torch.Tensor:
         """beta * beta_scale * sum_s w_rs phi1(f_r, f_s) * FOV_scale;
@@ -68,8 +80,8 @@ class RelativeDifferencePrior(NearestNeighbourPrior):
             _lib.dtype_code(dt), f.data_ptr(), out.data_ptr(), nx, ny, nz, float(m.dx), float(m.dy),
             float(m.dz), float(self.beta if beta is None else beta), float(self.gamma),
             float(self.delta), float(beta_scale), _lib.stream_ptr(f.device)))
-        fov = getattr(self, "FOV_scale", None)
-        if fov is not None and not bool(torch.all(fov == 1)):
+        fov = self._fov_factor()
+        if fov is not None:
             out = out * fov
         return out
 
@@ -83,8 +95,8 @@ class RelativeDifferencePrior(NearestNeighbourPrior):
         uu = _lib.require_device_tensor(u, "u", dt)
         if uu.shape != f.shape:
             raise ValueError(f"u shape {tuple(uu.shape)} != object shape {tuple(f.shape)}")
-        fov = getattr(self, "FOV_scale", None)
-        if fov is not None and not bool(torch.all(fov == 1)):
+        fov = self._fov_factor()
+        if fov is not None:
             uu = (uu * fov).contiguous()
         out = torch.empty_like(f)
         m = self.object_meta
@@ -112,8 +124,8 @@ class RelativeDifferencePrior(NearestNeighbourPrior):
             _lib.dtype_code(dt), f.data_ptr(), uu.data_ptr(), out.data_ptr(), nx, ny, nz,
             float(m.dx), float(m.dy), float(m.dz), float(self.beta if beta is None else beta),
             float(self.gamma), float(self.delta), float(beta_scale), _lib.stream_ptr(f.device)))
-        fov = getattr(self, "FOV_scale", None)
-        if fov is not None and not bool(torch.all(fov == 1)):
+        fov = self._fov_factor()
+        if fov is not None:
             out = out * fov
         return out
 
