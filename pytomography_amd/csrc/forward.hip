@@ -307,15 +307,24 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   // latency hides behind the blur phases of the current plane
   constexpr bool PF = AF;
   R Ar[PF ? NPA : 1][ZV];
-  auto issue = [&](int ip, int bb) {
-    if (!aact) return;
+  // The 16-byte / element-wise choice of load_zv is made once per phase
+  // (VEC is a compile-time constant inside the phase bodies): a per-load
+  // runtime branch would make the compiler wait for every outstanding load at
+  // each branch merge and serialise the independent gathers.
+  auto issue_v = [&](int ip, auto vec_c) {
+    constexpr bool VEC = decltype(vec_c)::value;
 #pragma unroll
     for (int p = 0; p < NPA; ++p) {
       const int jj = aco + p * cpp;
       if (jj < tj)
         load_zv<R, ZV>(attf + (((size_t)ang * g.P + ip) * n_jext + (j0 - g.je0 + jj)) * Nz, za, Nz,
-                       zvec, Ar[PF ? p : 0]);
+                       VEC, Ar[PF ? p : 0]);
     }
+  };
+  auto issue = [&](int ip, int bb) {
+    if (!aact) return;
+    if (zvec) issue_v(ip, std::true_type{});
+    else issue_v(ip, std::false_type{});
   };
   if constexpr (PF) {
     if (i >= 0) issue(i, 0);
@@ -325,8 +334,61 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
     const int inext = rem ? i_lo + 63 - __clzll(rem) : -1;
     if (inext >= 0) rem &= ~(1ull << (inext - i_lo));
     // ---- A: V = a * f_rot, ZV z per lane, NPA columns per thread
-    if (aact) {
+    auto phase_a = [&](auto vec_c) {
+      constexpr bool VEC = decltype(vec_c)::value;
       R *dstA = (H == 0 ? Vz : V + H) + za;
+      if constexpr (VEC && !TRB) {
+        // the 4 x GA tap loads of GA passes are issued before the first is
+        // used (all NPA passes for the float attenuation-field kernel, one
+        // pass otherwise, to stay within the register budget); columns past the
+        // tile load a valid column's taps and are not stored, so no divergent
+        // branch separates the loads
+        constexpr int GA = (sizeof(R) == 4 && AF) ? NPA : 1;
+#pragma unroll
+        for (int p0 = 0; p0 < NPA; p0 += GA) {
+        R raw[GA][4][ZV];
+        Taps4<R> tt[GA];
+#pragma unroll
+        for (int g = 0; g < GA; ++g) {
+          const int jc = min(aco + (p0 + g) * cpp, tj - 1);
+          tt[g] = tp[b][jc];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) load_zv<R, ZV>(f + tt[g].o[q], za, Nz, true, raw[g][q]);
+        }
+#pragma unroll
+        for (int g = 0; g < GA; ++g) {
+          const int p = p0 + g;
+          const int jj = aco + p * cpp;
+          R val[ZV];
+#pragma unroll
+          for (int e = 0; e < ZV; ++e) {
+            R a = raw[g][0][e] * tt[g].w[0];
+            a = a + raw[g][1][e] * tt[g].w[1];
+            a = a + raw[g][2][e] * tt[g].w[2];
+            a = a + raw[g][3][e] * tt[g].w[3];
+            val[e] = a;
+          }
+          if constexpr (PF) {
+#pragma unroll
+            for (int e = 0; e < ZV; ++e) val[e] = val[e] * Ar[p][e];
+          } else if (has_mu) {
+            R x[ZV];
+            interp_zv<R, ZV>(mu, tt[g], za, Nz, true, x);
+#pragma unroll
+            for (int e = 0; e < ZV; ++e) {
+              const R xm = x[e] * J.g.dx;
+              S[AF ? 0 : p][e] += (double)xm;
+              val[e] = val[e] * att_factor<R>(S[AF ? 0 : p][e], xm);
+            }
+          }
+          if (jj < tj) {
+#pragma unroll
+            for (int e = 0; e < ZV; ++e) dstA[(size_t)jj * ldv + e] = val[e];
+          }
+        }
+        }
+        return;
+      }
 #pragma unroll
       for (int p = 0; p < NPA; ++p) {
         const int jj = aco + p * cpp;
@@ -339,13 +401,13 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
             const int cnt = trb_n[b][jj];
             for (int q = 0; q < cnt; ++q) {
               R v[ZV];
-              load_zv<R, ZV>(f + trb_off[b][jj][q], za, Nz, zvec, v);
+              load_zv<R, ZV>(f + trb_off[b][jj][q], za, Nz, VEC, v);
               const R w = trb_w[b][jj][q];
 #pragma unroll
               for (int e = 0; e < ZV; ++e) val[e] += v[e] * w;
             }
           } else {
-            interp_zv<R, ZV>(f, t, za, Nz, zvec, val);
+            interp_zv<R, ZV>(f, t, za, Nz, VEC, val);
           }
           if constexpr (PF) {
 #pragma unroll
@@ -353,12 +415,12 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
           } else if constexpr (AF) {
             R a[ZV];
             load_zv<R, ZV>(attf + (((size_t)ang * g.P + i) * n_jext + (j0 - g.je0 + jj)) * Nz, za,
-                           Nz, zvec, a);
+                           Nz, VEC, a);
 #pragma unroll
             for (int e = 0; e < ZV; ++e) val[e] = val[e] * a[e];
           } else if (has_mu) {
             R x[ZV];
-            interp_zv<R, ZV>(mu, t, za, Nz, zvec, x);
+            interp_zv<R, ZV>(mu, t, za, Nz, VEC, x);
 #pragma unroll
             for (int e = 0; e < ZV; ++e) {
               const R xm = x[e] * g.dx;
@@ -368,9 +430,13 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
           }
 #pragma unroll
           for (int e = 0; e < ZV; ++e)
-            if (za + e < Nz) dstA[(size_t)jj * ldv + e] = val[e];
+            if (VEC || za + e < Nz) dstA[(size_t)jj * ldv + e] = val[e];
         }
       }
+    };
+    if (aact) {
+      if (zvec) phase_a(std::true_type{});
+      else phase_a(std::false_type{});
     }
     if (inext >= 0 && tid < tj) setup(inext, b ^ 1);
     if constexpr (H == 0) {
