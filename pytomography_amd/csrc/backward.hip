@@ -118,6 +118,17 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
     }
     if (!__syncthreads_or(inside)) continue;
     R bz[RZ];
+    // AF: this plane's attenuation factors are requested before the z blur
+    // (after the y-blur barrier, which would otherwise wait for them) so that
+    // their HBM latency overlaps the blur arithmetic
+    R a[AF ? RZ : 1];
+    auto load_att = [&]() {
+      if constexpr (AF) {
+        if (mine)
+          load_run<R, RZ>(attf + (((size_t)ang * g.P + i) * n_jext + (j0 - g.je0 + jl)) * Nz, z0,
+                          Nz, a);
+      }
+    };
     if constexpr (H > 0) {
       // ---- A: y blur of the projection with this plane's taps -> U
       if (aact) {
@@ -135,6 +146,7 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
         for (int r = 0; r < HALF_ROWS; ++r) U[(size_t)(half * HALF_ROWS + r) * ldu + H + za] = u[r];
       }
       __syncthreads();
+      load_att();
       // ---- B: z blur of the own run
       if (bact) {
         const R *wz = Wt + (size_t)(i - i_lo) * 2 * K;
@@ -152,6 +164,7 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
         }
       }
     } else {
+      load_att();
       // no PSF: the smeared projection itself
       if (bact) {
         const int j = j0 + jl;
@@ -167,9 +180,6 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
     if (mine) {
       Taps4<R> t = taps4<R>(src[buf][jl], g.N, Nz);
       if constexpr (AF) {
-        R a[RZ];
-        load_run<R, RZ>(attf + (((size_t)ang * g.P + i) * n_jext + (j0 - g.je0 + jl)) * Nz, z0, Nz,
-                        a);
 #pragma unroll
         for (int m = 0; m < RZ; ++m) bz[m] = bz[m] * a[m];
       } else if (has_mu) {
@@ -297,13 +307,55 @@ __device__ __forceinline__ ZVec<R, ZV> load_zvec(const R *p) {
 }
 
 template <typename R, int ZV>
+__device__ __forceinline__ void store_zvec(R *p, const R (&v)[ZV]) {
+  if constexpr (sizeof(R) == 4 && ZV == 4) {
+    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  } else if constexpr (sizeof(R) == 8 && ZV == 2) {
+    *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+  } else {
+#pragma unroll
+    for (int m = 0; m < ZV; ++m) p[m] = v[m];
+  }
+}
+
+// VEC: all ZV values valid and 16-byte aligned; the operands of the update are
+// read as vectors (one request each instead of one per value).
+template <typename R, int ZV, bool VEC = false>
 __device__ __forceinline__ void bp_store_vec(int out_mode, R *__restrict__ out, size_t o,
                                              const R (&b)[ZV], int nvalid,
                                              const R *__restrict__ f, const R *__restrict__ nrm,
                                              R delta, R floor_value) {
+#pragma clang fp contract(off)
+  if constexpr (VEC) {
+    R r[ZV];
+    if (out_mode == SPECT_OUT_STORE) {
 #pragma unroll
-  for (int m = 0; m < ZV; ++m)
-    if (m < nvalid) bp_store<R>(out_mode, out, o + m, b[m], f, nrm, delta, floor_value);
+      for (int m = 0; m < ZV; ++m) r[m] = b[m];
+    } else if (out_mode == SPECT_OUT_ADD) {
+      const ZVec<R, ZV> ov = load_zvec<R, ZV>(out + o);
+#pragma unroll
+      for (int m = 0; m < ZV; ++m) r[m] = ov.v[m] + b[m];
+    } else {
+      const ZVec<R, ZV> nv = load_zvec<R, ZV>(nrm + o);
+      const ZVec<R, ZV> fv = load_zvec<R, ZV>(f + o);
+#pragma unroll
+      for (int m = 0; m < ZV; ++m) {
+        // the same expression as bp_store
+        const R n = nv.v[m];
+        const R fo = fv.v[m];
+        const R pre = fo * (R(1) / (n + delta));
+        const R grad = b[m] - n;
+        R v = fo + pre * (grad - R(0));
+        if (v <= floor_value) v = floor_value;
+        r[m] = v;
+      }
+    }
+    store_zvec<R, ZV>(out + o, r);
+  } else {
+#pragma unroll
+    for (int m = 0; m < ZV; ++m)
+      if (m < nvalid) bp_store<R>(out_mode, out, o + m, b[m], f, nrm, delta, floor_value);
+  }
 }
 
 // Each thread owns one object column (x, y) and ZV consecutive z; the wave's
@@ -344,8 +396,14 @@ __device__ __forceinline__ void gather_body(const GatherJob<R> &J, size_t gid) {
   for (int m = 0; m < ZV; ++m) acc[m] = R(0);
 #pragma unroll
   for (int m = 0; m < (MODE == 2 ? ZV : 1); ++m) accm[m] = 0.0;
+  // The vector / element choice is made once per thread (VEC is a
+  // compile-time constant inside run): a branch around every load would make
+  // the compiler wait for all outstanding loads at each merge.
+  const R linY = tb.lin[Y], linX = tb.lin[X];  // angle-independent
+  auto run = [&](auto vec_c) {
+  constexpr bool VEC = decltype(vec_c)::value;
   auto fetch = [&](const R *p, R (&v)[ZV]) {
-    if (vec) {
+    if constexpr (VEC) {
       const ZVec<R, ZV> t = load_zvec<R, ZV>(p);
 #pragma unroll
       for (int m = 0; m < ZV; ++m) v[m] = t.v[m];
@@ -361,7 +419,7 @@ __device__ __forceinline__ void gather_body(const GatherJob<R> &J, size_t gid) {
     const R *Wa = W + (size_t)al * g.P * plane;
     if constexpr (MODE == 0) {
       R ix, iy;
-      rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, ix, iy);  // theta_b = -theta_f
+      rot_coord<R>(cs.x, -cs.y, linY, linX, g.half, ix, iy);  // theta_b = -theta_f
       const R fx = floor(ix), fy = floor(iy);
       const int c0 = int(fx), r0 = int(fy);
       const R tx = ix - fx, ty = iy - fy;
@@ -391,7 +449,7 @@ __device__ __forceinline__ void gather_body(const GatherJob<R> &J, size_t gid) {
     } else {
       // rotated-frame position whose forward sample lands on (row X, col Y)
       R jc, ic;
-      rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, jc, ic);
+      rot_coord<R>(cs.x, -cs.y, linY, linX, g.half, jc, ic);
       const int ib = int(floor(ic - R(1.5))), jb = int(floor(jc - R(1.5)));
       R v[ZV];
 #pragma unroll
@@ -433,8 +491,11 @@ __device__ __forceinline__ void gather_body(const GatherJob<R> &J, size_t gid) {
       for (int m = 0; m < ZV; ++m) acc[m] = acc[m] + v[m];
     }
   }
-  bp_store_vec<R, ZV>(out_mode, out, ((size_t)x * N + y) * Nz + z0, acc, nvalid, f, nrm, delta,
-                      floor_value);
+  bp_store_vec<R, ZV, VEC>(out_mode, out, ((size_t)x * N + y) * Nz + z0, acc, nvalid, f, nrm,
+                           delta, floor_value);
+  };
+  if (vec) run(std::true_type{});
+  else run(std::false_type{});
   if constexpr (MODE == 2) {
     R gm[ZV];
 #pragma unroll
