@@ -289,8 +289,8 @@ def bench_aux(args, device):
     """The HBM-streaming kernels beside the projector at 128^3 / 120 x 128 x 128:
     RDP gradient and its order-2 operators, the BSREM update, the Poisson
     ratio, the energy-window scatter estimate and its smoothing, the KEM basis
-    (size 5) and the DVF warp.  Each is timed with HIP events on the launch
-    stream over ``steps`` launches; achieved = algorithmic bytes / time."""
+    (size 5) and the DVF warp.  Each is timed with HIP events around a graph
+    replay of ``steps`` launches; achieved = algorithmic bytes / time."""
     import pytomography_amd
     from pytomography_amd import _lib
     from pytomography_amd.metadata.SPECT import SPECTObjectMeta, SPECTProjMeta
@@ -349,15 +349,27 @@ def bench_aux(args, device):
     ]
     rows = []
     for name, fn, nbytes in cases:
-        for _ in range(args.warmup):
-            fn()
+        # the launches are captured in a hipGraph and replayed, so the time is the
+        # kernels' and not the Python call overhead between them
+        side = torch.cuda.Stream(device)
+        side.wait_stream(stream)
+        with torch.cuda.stream(side):
+            for _ in range(args.warmup):
+                fn()
+        stream.wait_stream(side)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for _ in range(args.steps):
+                fn()
+        graph.replay()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         e0.record(stream)
-        for _ in range(args.steps):
-            fn()
+        graph.replay()
         e1.record(stream)
         torch.cuda.synchronize()
+        del graph
         us = e0.elapsed_time(e1) * 1e3 / args.steps
         gbs = nbytes / (us * 1e-6) / 1e9
         rows.append({"kernel": name, "us": us, "algorithmic_bytes": nbytes, "achieved_GBs": gbs,
