@@ -398,13 +398,24 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
 #pragma unroll
           for (int e = 0; e < ZV; ++e) val[e] = R(0);
           if constexpr (TRB) {
+            // contributors in groups of 4 whose loads are issued together;
+            // slots past cnt re-read contributor 0 with weight 0 (no branch
+            // between the loads); the sum keeps the contributor order
             const int cnt = trb_n[b][jj];
-            for (int q = 0; q < cnt; ++q) {
-              R v[ZV];
-              load_zv<R, ZV>(f + trb_off[b][jj][q], za, Nz, VEC, v);
-              const R w = trb_w[b][jj][q];
+            for (int q0 = 0; q0 < cnt; q0 += 4) {
+              R v[4][ZV];
+              R w[4];
 #pragma unroll
-              for (int e = 0; e < ZV; ++e) val[e] += v[e] * w;
+              for (int u = 0; u < 4; ++u) {
+                const bool use = q0 + u < cnt;
+                const int q = use ? q0 + u : 0;
+                w[u] = use ? trb_w[b][jj][q] : R(0);
+                load_zv<R, ZV>(f + trb_off[b][jj][q], za, Nz, VEC, v[u]);
+              }
+#pragma unroll
+              for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int e = 0; e < ZV; ++e) val[e] += v[u][e] * w[u];
             }
           } else {
             interp_zv<R, ZV>(f, t, za, Nz, VEC, val);
