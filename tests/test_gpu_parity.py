@@ -22,6 +22,8 @@ CASES = [
     dict(n=24, n_angles=12, att=False, psf=True),
     dict(n=32, n_angles=12, att=True, psf=True),
     dict(n=33, n_angles=10, att=True, psf=True, nz=20),
+    # z extent not a multiple of 4: the element-wise (ragged) load paths
+    dict(n=21, n_angles=9, att=True, psf=True, nz=13),
 ]
 
 
@@ -99,6 +101,24 @@ def test_adjoint_identity_fp64(cuda, case):
     print(f"[{_ids(case)}] <Hx,y>={lhs:.12e} <x,HTy>={rhs:.12e} rel={rel:.2e}; "
           f"reference backward mismatch {rel_ref:.2e}")
     assert rel <= 1e-6
+
+
+def test_ragged_z_adjoint_and_identity(cuda):
+    """z extent 13 (not a multiple of 4): the exact adjoint's element-wise
+    gather path against the oracle's autograd, and <Hx,y> = <x,H^T y> in FP64."""
+    pa = problem_arrays(21, 9, seed=9, nz=13, att=True, psf=True)
+    sm = product_system_matrix(pa)
+    gen = torch.Generator().manual_seed(10)
+    y = torch.rand((9, 21, 13), generator=gen, dtype=torch.float64)
+    x = torch.rand((21, 21, 13), generator=gen, dtype=torch.float64)
+    ref = so.exact_adjoint(oracle_geometry(pa, torch.float64), y)
+    e32 = rel_l2(sm.backward_adjoint(y.float().to(cuda)).cpu(), ref)
+    hx = sm.forward(x.to(cuda))
+    hty = sm.backward_adjoint(y.to(cuda))
+    lhs, rhs = float((hx * y.to(cuda)).sum()), float((x.to(cuda) * hty).sum())
+    rel = abs(lhs - rhs) / abs(lhs)
+    print(f"ragged z: exact adjoint FP32 vs FP64 oracle {e32:.2e}; adjoint identity {rel:.2e}")
+    assert e32 <= TOL and rel <= 1e-12
 
 
 @pytest.mark.parametrize("nz", [24, 160])

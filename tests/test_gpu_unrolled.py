@@ -30,6 +30,7 @@ CASES = [
     dict(n=16, n_angles=8, att=False, psf=False),
     dict(n=24, n_angles=12, att=True, psf=True),
     dict(n=33, n_angles=10, att=True, psf=True, nz=20),
+    dict(n=21, n_angles=9, att=True, psf=True, nz=13),
 ]
 
 
