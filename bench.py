@@ -485,6 +485,8 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     its = args.steps / elapsed
+    f32_iterate = alg.object_prediction.detach().clone()
+    n_done = args.warmup + args.steps
 
     # Per-entry-point timing with HIP events on the launch stream (one more iteration).
     stream = torch.cuda.current_stream(device)
@@ -510,6 +512,24 @@ def main():
     kern, bytes_launch, ms = "fwd_kernel + fwd_reduce_kernel (spect_forward, 15 angles)", fwd_b, fwd_ms
     achieved = bytes_launch / (ms * 1e-3) / 1e9
     iter_bytes = ns * (fwd_b + bwd_b + upd_b + ratio_b)
+
+    # The same number of iterations with the float64 kernels (untimed): the
+    # float32 iterate's error against an FP64 run of the same path.
+    import pytomography_amd
+    pytomography_amd.set_dtype(torch.float64)
+    try:
+        lk64 = PoissonLogLikelihood(sm, prob["data"].double(), additive_term=prob["scatter"])
+        lk64._set_n_subsets(ns)
+        alg64 = OSEM(lk64)
+        alg64._set_n_subsets = lambda k: None
+        for _ in range(n_done):
+            for k in range(ns):
+                alg64._fused_subiteration(0, k)
+        ref64 = alg64.object_prediction
+        fp64_err = float((f32_iterate.double() - ref64).norm() / ref64.norm())
+        del lk64, alg64, ref64
+    finally:
+        pytomography_amd.set_dtype(torch.float32)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -545,6 +565,8 @@ def main():
             "iteration_algorithmic_bytes": iter_bytes,
             "iteration_hbm_frac": iter_bytes * its / (HBM_PEAK_GBS * 1e9),
             "sensitivity_fp64_s": sens_s,
+            "iterate_rel_l2_f32_vs_f64": fp64_err,
+            "iterations_compared": n_done,
             "forward_ms_per_subset": fwd_ms,
             "backward_update_ms_per_subset": bwd_ms,
         },
