@@ -48,6 +48,21 @@ def test_c_abi_rejects_bad_arguments_without_touching_the_gpu():
     assert lib.spect_forward(None, 0, None, None, None, None, 1, None, None, 0, None) == -1
     assert lib.spect_workspace_bytes(None, 0, 4) == 0
     assert lib.spect_poisson_ratio(0, None, None, None, None, None, 10, 1e-11, None) == -1
+    # argument checks of the auxiliary entry points run before any launch
+    fake = ctypes.c_void_p(0x1000)
+    fake2 = ctypes.c_void_p(0x2000)
+    assert lib.spect_smooth_axis(0, fake, fake2, 4, 4, 4, 1, fake, 4, 1, None) == -1
+    assert b"odd" in lib.spect_last_error()
+    assert lib.spect_smooth_axis(0, fake, fake, 4, 4, 4, 1, fake2, 5, 1, None) == -1
+    assert lib.spect_smooth_axis(0, fake, fake2, 4, 4, 4, 3, fake, 5, 1, None) == -1
+    assert lib.spect_smooth_axis(0, fake, fake2, 4, 4, 4, 1, fake, 5, 7, None) == -1
+    assert lib.spect_ew_scatter(0, fake, None, fake2, None, 10, 0.0, 1.0, 1.0, 0.5, 0.5, None) == -1
+    assert b"non-zero" in lib.spect_last_error()
+    assert lib.spect_kem_apply(0, fake, fake2, fake, 4, 4, 4, 4, None) == -1
+    assert lib.spect_kem_apply(0, fake, fake2, fake2, 4, 4, 4, 3, None) == -1
+    assert lib.spect_dvf_warp(0, fake, fake2, fake, 1, 4, 4, None) == -1
+    assert lib.spect_rdp_hessian_jvp(0, fake, fake2, fake, 0, 4, 4, 1, 1, 1, 0.1, 2, 1e-11, 1, None) == -1
+    assert lib.spect_ew_scatter(7, fake, None, fake2, None, 10, 1.0, 1.0, 1.0, 0.5, 0.5, None) == -1
 
 
 def test_no_cpu_fallback():
