@@ -422,8 +422,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py measures one GPU only (asked for {max(args.gpus, world)}): this build "
-              "targets a single MI355X; the multi-GPU sharding is described in DESIGN.md.",
+        print(f"bench.py measures one GPU only (asked for {max(args.gpus, world)}): the "
+              "projectors run on a single MI355X; sharding across GPUs is not implemented.",
               file=sys.stderr)
         sys.exit(2)
     if not torch.cuda.is_available():
@@ -446,9 +446,8 @@ def main():
         # A run of this workload at 128^3 x 120 ended in torch.poisson's
         # non-negative-rate device assert while simulating the data; the
         # cause is not identified, so it stays off the GPU at that size.
-        print("bench.py: --workload plain above 64^3 x 60 is disabled (unresolved abort while "
-              "simulating the data; see the known-issues section of the design notes)",
-              file=sys.stderr)
+        print("bench.py: --workload plain above 64^3 x 60 is disabled (an abort while "
+              "simulating the data at 128^3 x 120 has no identified cause)", file=sys.stderr)
         sys.exit(2)
 
     from pytomography_amd.algorithms import OSEM
