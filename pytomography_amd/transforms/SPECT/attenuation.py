@@ -19,8 +19,8 @@ class SPECTAttenuationTransform(Transform):
     Args:
         attenuation_map: tensor [Lx, Ly, Lz] of attenuation coefficients (1/cm)
             aligned with the SPECT object (attenuation.py:40-48).
-        filepath: CT DICOM series -- not supported by this build (the CT -> mu
-            conversion lives outside the projector path).
+        filepath: CT DICOM series -- not supported (reading DICOM needs pydicom,
+            and the CT -> mu conversion lives outside the projector path).
         assume_padded: must be True (the reference default, attenuation.py:53):
             the kernels always work in the padded rotation frame.
     """
@@ -33,7 +33,7 @@ class SPECTAttenuationTransform(Transform):
             raise Exception("Please supply only one of `attenuation_map` or `filepath` as arguments")
         if filepath is not None:
             raise NotImplementedError(
-                "CT DICOM -> attenuation map conversion is not part of this build; "
+                "CT DICOM -> attenuation map conversion is not supported by pytomography_amd; "
                 "pass attenuation_map (1/cm) aligned with the object")
         if not assume_padded:
             raise NotImplementedError("assume_padded=False is not supported by the fused kernels")
