@@ -51,12 +51,13 @@ class KEMTransform(Transform):
         top-N selection, distance weighting, normalisation over the taps."""
         shape = self.support_objects[0].shape
         kernel = torch.ones((self.size, self.size, self.size, *shape)).to(pytomography_amd.dtype)
+        support = [so.cpu() for so in self.support_objects]
         for i in self.idxs:
             for j in self.idxs:
                 for k in self.idxs:
                     component = 1
-                    for l in range(len(self.support_objects)):
-                        so = self.support_objects[l].cpu()
+                    for l in range(len(support)):
+                        so = support[l]
                         nb = get_object_nearest_neighbour(so, (int(i), int(j), int(k)))
                         component *= self.support_kernels[l](so, nb, *self.support_kernel_params[l])
                     kernel[i + self.idx_max, j + self.idx_max, k + self.idx_max] = component
