@@ -6,6 +6,7 @@ result also reports its error against the float64 oracle.  Adjoint identity
 """
 from __future__ import annotations
 
+import numpy as np
 import pytest
 import torch
 
@@ -101,6 +102,23 @@ def test_adjoint_identity_fp64(cuda, case):
     print(f"[{_ids(case)}] <Hx,y>={lhs:.12e} <x,HTy>={rhs:.12e} rel={rel:.2e}; "
           f"reference backward mismatch {rel_ref:.2e}")
     assert rel <= 1e-6
+
+
+@pytest.mark.parametrize("chunk_planes", [8, 24])
+def test_noncircular_orbit_and_chunking(cuda, chunk_planes):
+    """A different detector radius at every angle (a non-circular orbit: one
+    PSF table per radius) and a non-default plane chunk, forward and
+    reference back projection against the oracle."""
+    pa = problem_arrays(24, 10, seed=11, nz=20, att=True, psf=True)
+    pa["radii"] = 20.0 + 6.0 * np.abs(np.sin(np.deg2rad(pa["angles"])))
+    sm = product_system_matrix(pa, chunk_planes=chunk_planes)
+    geo = oracle_geometry(pa, torch.float32)
+    f = torch.from_numpy(pa["f"])
+    y = torch.rand((10, 24, 20), generator=torch.Generator().manual_seed(12))
+    ef = max(_per_projection_rel(sm.forward(f.to(cuda)).cpu(), so.forward(geo, f)))
+    eb = rel_l2(sm.backward(y.to(cuda)).cpu(), so.backward(geo, y))
+    print(f"non-circular orbit, L={chunk_planes}: forward {ef:.2e}, backward {eb:.2e}")
+    assert ef <= TOL and eb <= TOL
 
 
 def test_ragged_z_adjoint_and_identity(cuda):
