@@ -73,10 +73,13 @@ class SPECTSystemMatrix(SystemMatrix):
         attenuation_field: keep the attenuation factor of every rotated-frame
             voxel of every angle resident in device memory (built once per
             attenuation map) instead of resampling mu in every projection.
-            None: on for float32 when it needs at most ``ATT_FIELD_AUTO_BYTES``.
+            None: on for float32 when it needs at most ``ATT_FIELD_AUTO_BYTES`` and at
+            most ``ATT_FIELD_AUTO_FRACTION`` of the device's memory (e.g. 19 GB at
+            256^3 x 180 angles).
     """
 
-    ATT_FIELD_AUTO_BYTES = 8 << 30
+    ATT_FIELD_AUTO_BYTES = 64 << 30
+    ATT_FIELD_AUTO_FRACTION = 0.25
 
     def __init__(self, obj2obj_transforms, proj2proj_transforms, object_meta, proj_meta,
                  object_initial_based_on_camera_path: bool = False,
@@ -215,7 +218,9 @@ class SPECTSystemMatrix(SystemMatrix):
         if dt != torch.float32:
             return False
         plan = self._plan(device)
-        return int(plan.lib.spect_att_field_bytes(plan.handle, _lib.dtype_code(dt))) <= self.ATT_FIELD_AUTO_BYTES
+        need = int(plan.lib.spect_att_field_bytes(plan.handle, _lib.dtype_code(dt)))
+        total = torch.cuda.get_device_properties(device).total_memory
+        return need <= min(self.ATT_FIELD_AUTO_BYTES, self.ATT_FIELD_AUTO_FRACTION * total)
 
     def _att_table(self, device, dt):
         """Attenuation data of the current mu, built once per map: the FP64
