@@ -19,9 +19,7 @@
 // Planes where no column of the tile samples the object are skipped.  Each
 // block writes its kTJ + 2H output columns; a reduce kernel adds chunks and
 // overlapping tiles in a fixed order (deterministic, no atomics).
-#include <type_traits>
-
-#include "launch.hpp"
+#include "forward_body.hpp"
 
 namespace spect {
 
@@ -141,400 +139,6 @@ __global__ __launch_bounds__(1024) void att_field_kernel(Geo<R> g, Tables<R> tb,
   }
 }
 
-// LDS rows: H zeros | runs*RZ values | H zeros | 1 pad (odd stride).
-template <int RZ, int H> __device__ __forceinline__ int lds_stride(int runs) {
-  return runs * RZ + 2 * H + 1;
-}
-
-// TRB: instead of the forward bilinear gather of f, the transpose of the
-// reference back projection's gather at theta_b = -theta_f
-// (spect_system_matrix.py:196): rotated-frame pixel (i, j) collects every
-// object pixel whose theta_b sample has (i, j) among its bilinear taps, with
-// that tap's weight -- the body then computes B^T v (attenuation, PSF, sum).
-constexpr int kTrbMax = 12;  // contributors per rotated-frame pixel (<= 9 in exact arithmetic)
-
-template <typename R, int RZ, int QS, int H, bool TRB, bool AF>
-__device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int c,
-                                         unsigned char *smem) {
-  constexpr int K = 2 * H + 1;
-  constexpr int JO = kTJ + 2 * H;      // output columns touched by the tile
-  constexpr int JH = (JO + QS - 1) / QS;  // per split
-  // bilinear taps per tile column, double buffered: the next plane's are
-  // built while the current plane is gathered
-  __shared__ Taps4<R> tp[2][kTJ];
-  __shared__ uint32_t trb_off[TRB ? 2 : 1][TRB ? kTJ : 1][kTrbMax];
-  __shared__ R trb_w[TRB ? 2 : 1][TRB ? kTJ : 1][kTrbMax];
-  __shared__ int trb_n[TRB ? 2 : 1][TRB ? kTJ : 1];
-  __shared__ unsigned long long act_s;  // planes of the chunk with a non-empty footprint
-  const Geo<R> &g = J.g;
-  const Tables<R> &tb = J.tb;
-  const R *__restrict__ f = J.f;
-  const R *__restrict__ mu = J.mu;
-  const double *__restrict__ att = J.att;
-  const int att_global = J.att_global;
-  const R *__restrict__ attf = J.attf;  // AF: precomputed attenuation factors
-  const int32_t *__restrict__ angle_idx = J.angle_idx;
-  R *__restrict__ partial = J.partial;
-  const int n_call = J.n;
-  const int ang = ld_uniform(angle_idx + al);
-  if ((unsigned)ang >= (unsigned)g.A) return;
-  const int tid = threadIdx.x;
-  const int Nz = g.Nz;
-  const int runs = (Nz + RZ - 1) / RZ;
-  const int ldv = lds_stride<RZ, H>(runs);
-  R *V = reinterpret_cast<R *>(smem);         // [kTJ][ldv], data at +H
-  R *Vz = V + (size_t)kTJ * ldv;              // [kTJ][ldv], data at +0
-  const int j0 = g.je0 + jt * kTJ;
-  const int tj = min(kTJ, g.je1 - j0);
-  const int n_jext = g.je1 - g.je0;
-  // phase B ownership: column jl, z run [z0, z0 + RZ)
-  const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
-  const bool ab = run < runs;
-  // phase A ownership: ZV consecutive z (one 16-byte load) per lane with the
-  // lanes along z, so every wave load covers whole 64-byte sectors; the
-  // thread covers columns aco + p * cpp, p < NPA (cpp * NPA >= kTJ is
-  // checked at launch)
-  constexpr int ZV = 16 / sizeof(R);
-  constexpr int NPA = phase_a_passes<R, QS>();
-  const int nq = (Nz + ZV - 1) / ZV;
-  const int cpp = blockDim.x / nq;
-  const int aco = tid / nq, za = (tid - aco * nq) * ZV;
-  const bool aact = aco < cpp;
-  const bool zvec = (Nz % ZV) == 0;
-  // phase C ownership: one z, 1/QS of the output columns (wave-uniform split)
-  const int nzc = (Nz + 63) / 64 * 64;
-  const int half = tid / nzc, zc = tid - half * nzc;
-  const bool cact = half < QS && zc < Nz;
-  const auto cs = ld_uniform(tb.trig + ang);
-  const bool has_mu = mu != nullptr;
-
-  // V and Vz start zeroed: halo pads, columns >= tj and z slots >= Nz are
-  // never written afterwards
-  for (int q = tid; q < 2 * kTJ * ldv; q += blockDim.x) V[q] = R(0);
-  double S[AF ? 1 : NPA][ZV];
-#pragma unroll
-  for (int p = 0; p < (AF ? 1 : NPA); ++p) {
-    const int jj = aco + p * cpp;
-#pragma unroll
-    for (int e = 0; e < ZV; ++e) {
-      S[p][e] = 0.0;
-      if (!AF && has_mu && aact && jj < tj && za + e < Nz) {
-        const int rowa = att_global ? ang : al;
-        S[p][e] = att[(((size_t)rowa * g.C + c) * n_jext + (j0 - g.je0 + jj)) * Nz + za + e];
-      }
-    }
-  }
-  R acc[JH];
-#pragma unroll
-  for (int q = 0; q < JH; ++q) acc[q] = R(0);
-
-  const int rix = ld_uniform(tb.radius_index + ang);
-  const int i_lo = c * g.L, i_hi = min(g.P, i_lo + g.L);
-  // the chunk's PSF taps (z then y, per plane) staged in LDS: the blur phases
-  // read them as LDS broadcasts instead of waiting on scalar loads
-  R *Wt = Vz + (size_t)kTJ * ldv;  // [i_hi - i_lo][2K]
-  if constexpr (H > 0) {
-    const int nt = (i_hi - i_lo) * 2 * K;
-    for (int q = tid; q < nt; q += blockDim.x) {
-      const int l = q / (2 * K), r = q - l * (2 * K);
-      const size_t row = ((size_t)rix * g.P + i_lo + l) * K;
-      Wt[q] = r < K ? tb.taps_z[row + r] : tb.taps_r[row + r - K];
-    }
-  }
-  // which planes of the chunk can touch the object (bit l: plane i_lo + l);
-  // the skipped planes sample outside the object, where f and mu vanish
-  if (tid == 0) act_s = 0ull;
-  __syncthreads();
-  for (int q = tid; q < (i_hi - i_lo) * kTJ; q += blockDim.x) {
-    const int l = q / kTJ, jj = q - l * kTJ;
-    if (jj < tj) {
-      R ix, iy;
-      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + jj], tb.lin[i_lo + l], g.half, ix, iy);
-      if (in_footprint<R>(ix, iy, g)) atomicOr(&act_s, 1ull << l);
-    }
-  }
-  const R lin_j = tid < tj ? tb.lin[j0 + tid] : R(0);
-  // taps (and for TRB the contributor lists) of plane i into buffer b; run by tid < tj
-  auto setup = [&](int i, int b) {
-    R ix, iy;
-    rot_coord<R>(cs.x, cs.y, lin_j, ld_uniform(tb.lin + i), g.half, ix, iy);
-    tp[b][tid] = taps4<R>(make_src<R>(ix, iy, g.pad), g.N, Nz);
-    if constexpr (TRB) {
-      // object pixels (X, Y) near the forward sample position of (i, j)
-      const int j = j0 + tid;
-      const int Xb = int(floor(iy - R(1.5))), Yb = int(floor(ix - R(1.5)));
-      int cnt = 0;
-      for (int dx_ = 0; dx_ < 4; ++dx_) {
-        const int X = Xb + dx_;
-        if ((unsigned)(X - g.pad) >= (unsigned)g.N) continue;
-        for (int dy_ = 0; dy_ < 4; ++dy_) {
-          const int Y = Yb + dy_;
-          if ((unsigned)(Y - g.pad) >= (unsigned)g.N) continue;
-          R jc, ic;
-          rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, jc, ic);
-          const R fjc = floor(jc), fic = floor(ic);
-          const int c0 = int(fjc), r0 = int(fic);
-          const R tx = jc - fjc, ty = ic - fic;
-          R wr, wc;
-          if (r0 == i) wr = R(1) - ty;
-          else if (r0 + 1 == i) wr = ty;
-          else continue;
-          if (c0 == j) wc = R(1) - tx;
-          else if (c0 + 1 == j) wc = tx;
-          else continue;
-          if (cnt < kTrbMax) {
-            trb_off[b][tid][cnt] = (uint32_t)((X - g.pad) * g.N + (Y - g.pad)) * (uint32_t)Nz;
-            trb_w[b][tid][cnt] = wr * wc;
-            ++cnt;
-          }
-        }
-      }
-      // a contributor's forward sample lies within sqrt(2) px of the object,
-      // inside the footprint square, so the plane is in act
-      trb_n[b][tid] = cnt;
-    }
-  };
-  __syncthreads();
-  unsigned long long rem = act_s;
-  int i = rem ? i_lo + 63 - __clzll(rem) : -1;
-  if (i >= 0) {
-    rem &= ~(1ull << (i - i_lo));
-    if (tid < tj) setup(i, 0);
-  }
-  __syncthreads();
-  // PF: the attenuation factors of the next plane (a coalesced stream from
-  // HBM) are loaded into registers right after the phase-A barrier, so their
-  // latency hides behind the blur phases of the current plane
-  constexpr bool PF = AF;
-  R Ar[PF ? NPA : 1][ZV];
-  // The 16-byte / element-wise choice of load_zv is made once per phase
-  // (VEC is a compile-time constant inside the phase bodies): a per-load
-  // runtime branch would make the compiler wait for every outstanding load at
-  // each branch merge and serialise the independent gathers.
-  auto issue_v = [&](int ip, auto vec_c) {
-    constexpr bool VEC = decltype(vec_c)::value;
-#pragma unroll
-    for (int p = 0; p < NPA; ++p) {
-      const int jj = aco + p * cpp;
-      if (jj < tj)
-        load_zv<R, ZV>(attf + (((size_t)ang * g.P + ip) * n_jext + (j0 - g.je0 + jj)) * Nz, za, Nz,
-                       VEC, Ar[PF ? p : 0]);
-    }
-  };
-  auto issue = [&](int ip, int bb) {
-    if (!aact) return;
-    if (zvec) issue_v(ip, std::true_type{});
-    else issue_v(ip, std::false_type{});
-  };
-  if constexpr (PF) {
-    if (i >= 0) issue(i, 0);
-  }
-  int b = 0;
-  while (i >= 0) {
-    const int inext = rem ? i_lo + 63 - __clzll(rem) : -1;
-    if (inext >= 0) rem &= ~(1ull << (inext - i_lo));
-    // ---- A: V = a * f_rot, ZV z per lane, NPA columns per thread
-    auto phase_a = [&](auto vec_c) {
-      constexpr bool VEC = decltype(vec_c)::value;
-      R *dstA = (H == 0 ? Vz : V + H) + za;
-      if constexpr (VEC && !TRB) {
-        // the 4 x GA tap loads of GA passes are issued before the first is
-        // used (all NPA passes for the float attenuation-field kernel, one
-        // pass otherwise, to stay within the register budget); columns past the
-        // tile load a valid column's taps and are not stored, so no divergent
-        // branch separates the loads
-        constexpr int GA = (sizeof(R) == 4 && AF) ? NPA : 1;
-#pragma unroll
-        for (int p0 = 0; p0 < NPA; p0 += GA) {
-        R raw[GA][4][ZV];
-        Taps4<R> tt[GA];
-#pragma unroll
-        for (int g = 0; g < GA; ++g) {
-          const int jc = min(aco + (p0 + g) * cpp, tj - 1);
-          tt[g] = tp[b][jc];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) load_zv<R, ZV>(f + tt[g].o[q], za, Nz, true, raw[g][q]);
-        }
-#pragma unroll
-        for (int g = 0; g < GA; ++g) {
-          const int p = p0 + g;
-          const int jj = aco + p * cpp;
-          R val[ZV];
-#pragma unroll
-          for (int e = 0; e < ZV; ++e) {
-            R a = raw[g][0][e] * tt[g].w[0];
-            a = a + raw[g][1][e] * tt[g].w[1];
-            a = a + raw[g][2][e] * tt[g].w[2];
-            a = a + raw[g][3][e] * tt[g].w[3];
-            val[e] = a;
-          }
-          if constexpr (PF) {
-#pragma unroll
-            for (int e = 0; e < ZV; ++e) val[e] = val[e] * Ar[p][e];
-          } else if (has_mu) {
-            R x[ZV];
-            interp_zv<R, ZV>(mu, tt[g], za, Nz, true, x);
-#pragma unroll
-            for (int e = 0; e < ZV; ++e) {
-              const R xm = x[e] * J.g.dx;
-              S[AF ? 0 : p][e] += (double)xm;
-              val[e] = val[e] * att_factor<R>(S[AF ? 0 : p][e], xm);
-            }
-          }
-          if (jj < tj) {
-#pragma unroll
-            for (int e = 0; e < ZV; ++e) dstA[(size_t)jj * ldv + e] = val[e];
-          }
-        }
-        }
-        return;
-      }
-#pragma unroll
-      for (int p = 0; p < NPA; ++p) {
-        const int jj = aco + p * cpp;
-        if (jj < tj) {
-          const Taps4<R> t = tp[b][jj];
-          R val[ZV];
-#pragma unroll
-          for (int e = 0; e < ZV; ++e) val[e] = R(0);
-          if constexpr (TRB) {
-            // contributors in groups of 4 whose loads are issued together;
-            // slots past cnt re-read contributor 0 with weight 0 (no branch
-            // between the loads); the sum keeps the contributor order
-            const int cnt = trb_n[b][jj];
-            for (int q0 = 0; q0 < cnt; q0 += 4) {
-              R v[4][ZV];
-              R w[4];
-#pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const bool use = q0 + u < cnt;
-                const int q = use ? q0 + u : 0;
-                w[u] = use ? trb_w[b][jj][q] : R(0);
-                load_zv<R, ZV>(f + trb_off[b][jj][q], za, Nz, VEC, v[u]);
-              }
-#pragma unroll
-              for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int e = 0; e < ZV; ++e) val[e] += v[u][e] * w[u];
-            }
-          } else {
-            interp_zv<R, ZV>(f, t, za, Nz, VEC, val);
-          }
-          if constexpr (PF) {
-#pragma unroll
-            for (int e = 0; e < ZV; ++e) val[e] = val[e] * Ar[p][e];
-          } else if constexpr (AF) {
-            R a[ZV];
-            load_zv<R, ZV>(attf + (((size_t)ang * g.P + i) * n_jext + (j0 - g.je0 + jj)) * Nz, za,
-                           Nz, VEC, a);
-#pragma unroll
-            for (int e = 0; e < ZV; ++e) val[e] = val[e] * a[e];
-          } else if (has_mu) {
-            R x[ZV];
-            interp_zv<R, ZV>(mu, t, za, Nz, VEC, x);
-#pragma unroll
-            for (int e = 0; e < ZV; ++e) {
-              const R xm = x[e] * g.dx;
-              S[AF ? 0 : p][e] += (double)xm;
-              val[e] = val[e] * att_factor<R>(S[AF ? 0 : p][e], xm);
-            }
-          }
-#pragma unroll
-          for (int e = 0; e < ZV; ++e)
-            if (VEC || za + e < Nz) dstA[(size_t)jj * ldv + e] = val[e];
-        }
-      }
-    };
-    if (aact) {
-      if (zvec) phase_a(std::true_type{});
-      else phase_a(std::false_type{});
-    }
-    if (inext >= 0 && tid < tj) setup(inext, b ^ 1);
-    if constexpr (H == 0) {
-      __syncthreads();
-      if constexpr (PF) {
-        if (inext >= 0) issue(inext, b ^ 1);
-      }
-      if (cact) {
-        // no PSF: the ray sum of column jl is the plain sum over planes
-#pragma unroll
-        for (int q = 0; q < JH; ++q) {
-          const int jo = half * JH + q;
-          if (jo < JO) acc[q] += Vz[(size_t)jo * ldv + zc];
-        }
-      }
-      __syncthreads();
-    } else {
-      __syncthreads();
-      if constexpr (PF) {
-        if (inext >= 0) issue(inext, b ^ 1);
-      }
-      // ---- B: z blur (correlation, zero padded) of the own run
-      const R *wz = Wt + (size_t)(i - i_lo) * 2 * K;
-      if (ab) {
-        R win[RZ + 2 * H];
-        const R *vr = V + (size_t)jl * ldv + z0;  // win[m] = V(z0 - H + m)
-#pragma unroll
-        for (int m = 0; m < RZ + 2 * H; ++m) win[m] = vr[m];
-        R out[RZ];
-#pragma unroll
-        for (int m = 0; m < RZ; ++m) out[m] = R(0);
-#pragma unroll
-        for (int t = 0; t < K; ++t) {
-          const R w = wz[t];
-#pragma unroll
-          for (int m = 0; m < RZ; ++m) out[m] += w * win[m + t];
-        }
-#pragma unroll
-        for (int m = 0; m < RZ; ++m) Vz[(size_t)jl * ldv + z0 + m] = out[m];
-      }
-      __syncthreads();
-      // ---- C: y blur as a scatter into the tile's output columns
-      if (cact) {
-        const R *wy = Wt + (size_t)(i - i_lo) * 2 * K + K;
-        auto scatter = [&](auto half_c) {
-          constexpr int HC = decltype(half_c)::value;
-          constexpr int jlo = HC * JH;
-          constexpr int jhi = (jlo + JH < JO) ? jlo + JH : JO;
-          constexpr int jv0 = (jlo - 2 * H > 0) ? jlo - 2 * H : 0;
-          constexpr int jv1 = jhi < kTJ ? jhi : kTJ;
-          R v[kTJ];
-#pragma unroll
-          for (int jv = jv0; jv < jv1; ++jv) v[jv] = Vz[(size_t)jv * ldv + zc];
-          // out[jo] += w[t] * in[jo + t - 2H] (tile-local); t ascending is jv ascending
-#pragma unroll
-          for (int t = 0; t < K; ++t) {
-            const R w = wy[t];
-#pragma unroll
-            for (int q = 0; q < JH; ++q) {
-              const int jo = jlo + q;
-              const int jv = jo + t - 2 * H;
-              if (jo < jhi && jv >= jv0 && jv < jv1) acc[q] += w * v[jv];
-            }
-          }
-        };
-        switch (half) {
-          case 0: scatter(std::integral_constant<int, 0>{}); break;
-          case 1: if constexpr (QS > 1) scatter(std::integral_constant<int, 1>{}); break;
-          case 2: if constexpr (QS > 2) scatter(std::integral_constant<int, 2>{}); break;
-          case 3: if constexpr (QS > 3) scatter(std::integral_constant<int, 3>{}); break;
-          default: break;
-        }
-      }
-    }
-    i = inext;
-    b ^= 1;
-  }
-  if (cact) {
-    R *dst = partial + ((((size_t)c * n_call + al) * g.ntiles + jt) * JO) * Nz;
-#pragma unroll
-    for (int q = 0; q < JH; ++q) {
-      const int jo = half * JH + q;
-      if (jo < JO) dst[(size_t)jo * Nz + zc] = acc[q];
-    }
-  }
-}
-
 template <typename R, int RZ, int QS, int H, bool TRB, bool AF>
 __global__ __launch_bounds__(512) void fwd_kernel(FwdJob<R> J) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -553,45 +157,6 @@ __global__ __launch_bounds__(512) void fwd_batch_kernel(const FwdJob<R> *__restr
   const int local = b - ld_uniform(starts + j);
   const int c = local % J.g.C, r = local / J.g.C;
   fwd_body<R, RZ, QS, H, false, AF>(J, r / J.g.ntiles, r % J.g.ntiles, c, smem);
-}
-
-template <typename R>
-__device__ __forceinline__ void reduce_body(const ReduceJob<R> &J, size_t idx) {
-  const Geo<R> &g = J.g;
-  const R *__restrict__ partial = J.partial;
-  const int n = J.n;
-  R *__restrict__ proj = J.proj;
-  const R *__restrict__ gm = J.gm;
-  const R *__restrict__ s = J.s;
-  R *__restrict__ pred = J.pred;
-  R *__restrict__ ratio = J.ratio;
-  const R delta = J.delta;
-  const int nout = g.jo1 - g.jo0;
-  const size_t count = (size_t)n * nout * g.Nz;
-  if (idx >= count) return;
-  const int z = idx % g.Nz;
-  const int jx = (idx / g.Nz) % nout;
-  const int al = idx / ((size_t)g.Nz * nout);
-  const int JO = kTJ + 2 * g.H;
-  const int j = g.jo0 + jx;
-  const int rel = j - g.je0 + g.H;  // column within tile t: rel - t*kTJ
-  int t_hi = rel / kTJ;
-  if (t_hi > g.ntiles - 1) t_hi = g.ntiles - 1;
-  int t_lo = (rel - JO) < 0 ? 0 : (rel - JO) / kTJ + 1;
-  R acc = R(0);
-  for (int c = 0; c < g.C; ++c) {
-    for (int t = t_lo; t <= t_hi; ++t) {
-      const int jo = rel - t * kTJ;
-      acc += partial[((((size_t)c * n + al) * g.ntiles + t) * JO + jo) * g.Nz + z];
-    }
-  }
-  if (proj) proj[idx] = acc;
-  if (gm) {
-    R p = acc;
-    if (s) p = p + s[idx];
-    if (pred) pred[idx] = p;
-    ratio[idx] = gm[idx] / (p + delta);
-  }
 }
 
 template <typename R>

@@ -1,0 +1,51 @@
+// Host stand-in for <hip/hip_runtime.h> used by the CPU build of the forward
+// projector's block bodies (tests/cpu_build/fwd_cpu.cpp): the device
+// qualifiers compile away, __shared__ arrays become function statics (one
+// block runs at a time), every GPU lane is a std::thread and __syncthreads
+// is a std::barrier over the block's threads.
+#pragma once
+#include <algorithm>
+#include <barrier>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+
+using std::max;
+using std::min;
+
+#define __device__
+#define __host__
+#define __global__
+#define __forceinline__ inline
+#define __launch_bounds__(x)
+#define __shared__ static
+#define __align__(x) alignas(x)
+
+struct cpu_dim3 {
+  unsigned x = 0, y = 0, z = 0;
+};
+extern thread_local cpu_dim3 threadIdx;
+extern cpu_dim3 blockDim;
+extern std::barrier<> *cpu_block_barrier;
+
+inline void __syncthreads() { cpu_block_barrier->arrive_and_wait(); }
+inline unsigned long long atomicOr(unsigned long long *p, unsigned long long v) {
+  return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST);
+}
+inline int __clzll(unsigned long long v) { return __builtin_clzll(v); }
+inline float __expf(float v) { return std::exp(v); }
+
+struct float2 {
+  float x, y;
+};
+struct double2 {
+  double x, y;
+};
+struct alignas(16) float4 {
+  float x, y, z, w;
+};
+inline float2 make_float2(float a, float b) { return {a, b}; }
+inline double2 make_double2(double a, double b) { return {a, b}; }
+
+typedef int hipError_t;
+typedef void *hipStream_t;
