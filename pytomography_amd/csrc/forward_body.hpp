@@ -428,12 +428,32 @@ __device__ __forceinline__ void reduce_body(const ReduceJob<R> &J, size_t idx) {
   int t_hi = rel / kTJ;
   if (t_hi > g.ntiles - 1) t_hi = g.ntiles - 1;
   int t_lo = (rel - JO) < 0 ? 0 : (rel - JO) / kTJ + 1;
+  // The partials are added in the order (chunk ascending, tile ascending).
+  // kRB chunks x kRT tiles of loads are issued before the first add; a slot
+  // past the chunk count or the column's last tile re-reads a valid entry and
+  // is not added, so no branch separates the loads.  A column lies in at most
+  // ceil((kTJ + 2H) / kTJ) <= 3 tiles (H <= 20).
+  constexpr int kRB = 8, kRT = 3;
+  const size_t plane = (size_t)JO * g.Nz;
+  const size_t base0 = (size_t)al * g.ntiles * plane + (size_t)rel * g.Nz + z;
   R acc = R(0);
-  for (int c = 0; c < g.C; ++c) {
-    for (int t = t_lo; t <= t_hi; ++t) {
-      const int jo = rel - t * kTJ;
-      acc += partial[((((size_t)c * n + al) * g.ntiles + t) * JO + jo) * g.Nz + z];
+  for (int c0 = 0; c0 < g.C; c0 += kRB) {
+    R v[kRB][kRT];
+#pragma unroll
+    for (int u = 0; u < kRB; ++u) {
+      const int c = min(c0 + u, g.C - 1);
+#pragma unroll
+      for (int k = 0; k < kRT; ++k) {
+        const int t = min(t_lo + k, t_hi);
+        // entry (c, al, t, jo = rel - t*kTJ, z)
+        v[u][k] = partial[((size_t)c * n * g.ntiles + t) * plane + base0 - (size_t)t * kTJ * g.Nz];
+      }
     }
+#pragma unroll
+    for (int u = 0; u < kRB; ++u)
+#pragma unroll
+      for (int k = 0; k < kRT; ++k)
+        if (c0 + u < g.C && t_lo + k <= t_hi) acc += v[u][k];
   }
   if (proj) proj[idx] = acc;
   if (gm) {
