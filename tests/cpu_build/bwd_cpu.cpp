@@ -1,0 +1,75 @@
+// CPU build of the back projection: field_body and gather_body from
+// pytomography_amd/csrc/backward_body.hpp compiled with the host stand-in in
+// tests/cpu_build/hip, one std::thread per GPU lane for the field blocks.
+// The field W starts as NaN, as do the LDS buffers, so that a read of an entry
+// the field kernel never wrote -- also one taken with weight zero -- shows in
+// the output.
+//
+//   bwd_cpu <dir>     (problem files as in cpu_runtime.hpp, plus p.bin
+//                      float32 [n_sel][N][Nz], the projections of the
+//                      selected angles); writes <dir>/bp_ref.bin (reference
+//                      back projection, gather mode 0) and <dir>/bp_adj.bin
+//                      (exact adjoint, mode 1), float32 [N][N][Nz], and
+//                      prints "H .. w_written .. w_total .."
+#include "cpu_runtime.hpp"
+#include "backward_body.hpp"
+
+using namespace spect;
+
+int main(int argc, char **argv) {
+  if (argc != 2) {
+    std::fprintf(stderr, "usage: bwd_cpu <dir>\n");
+    return 2;
+  }
+  const std::string dir = argv[1];
+  const cpu_build::Problem p = cpu_build::load_problem(dir);
+  const Geo<float> &g = p.g;
+  const int N = p.N, Nz = p.Nz, n = p.n, H = p.H;
+  const auto proj = cpu_build::read_file<float>(dir + "/p.bin", (size_t)n * N * Nz);
+  const size_t n_jext = g.je1 - g.je0;
+  std::vector<float> W((size_t)n * p.P * n_jext * Nz, std::numeric_limits<float>::quiet_NaN());
+  const FieldJob<float> FJ{g, p.tables(), proj.data(), nullptr, nullptr, 0, nullptr, p.sel.data(), n,
+                           W.data(), nullptr, nullptr, nullptr};
+
+  // launch_field_h: block size and dynamic LDS
+  constexpr int RZ = BpTile<float>::RZ, QS = BpTile<float>::QS;
+  const int bt = cpu_build::block_threads(Nz, RZ, QS);
+  if (bt > 1024) {
+    std::fprintf(stderr, "launch configuration rejected\n");
+    return 2;
+  }
+  const int runs = (Nz + RZ - 1) / RZ;
+  std::vector<unsigned char> smem(
+      ((size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)p.L * 2 * (2 * H + 1)) * sizeof(float));
+  auto run = [&](auto h_c) {
+    constexpr int HH = decltype(h_c)::value;
+    cpu_build::run_grid(n, g.ntiles, p.C, bt, smem, [&](int al, int jt, int c, unsigned char *s) {
+      field_body<float, RZ, QS, HH, false, false>(FJ, al, jt, c, s);
+    });
+  };
+  switch (H) {
+    case 0: run(std::integral_constant<int, 0>{}); break;
+    case 2: run(std::integral_constant<int, 2>{}); break;
+    case 4: run(std::integral_constant<int, 4>{}); break;
+    case 6: run(std::integral_constant<int, 6>{}); break;
+    case 8: run(std::integral_constant<int, 8>{}); break;
+    default: std::fprintf(stderr, "PSF half width %d not built here\n", H); return 2;
+  }
+  const size_t w_written = W.size() - cpu_build::count_nan(W);
+
+  // launch_bp_gather: one lane per (x, y, 4 z); no barriers
+  constexpr int ZV = 4;
+  const size_t lanes = (size_t)N * N * ((Nz + ZV - 1) / ZV);
+  for (int mode = 0; mode < 2; ++mode) {
+    std::vector<float> out((size_t)N * N * Nz, std::numeric_limits<float>::quiet_NaN());
+    const GatherJob<float> GJ{g, p.tables(), W.data(), p.sel.data(), n, SPECT_OUT_STORE, out.data(),
+                              nullptr, nullptr, 0.0f, 0.0f, nullptr, nullptr, nullptr};
+    for (size_t gid = 0; gid < lanes; ++gid) {
+      if (mode == 0) gather_body<float, ZV, 0>(GJ, gid);
+      else gather_body<float, ZV, 1>(GJ, gid);
+    }
+    cpu_build::write_file(dir + (mode == 0 ? "/bp_ref.bin" : "/bp_adj.bin"), out);
+  }
+  std::printf("H %d w_written %zu w_total %zu\n", H, w_written, W.size());
+  return 0;
+}

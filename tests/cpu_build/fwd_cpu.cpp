@@ -2,98 +2,16 @@
 // pytomography_amd/csrc/forward_body.hpp compiled with the host stand-in in
 // tests/cpu_build/hip, one std::thread per GPU lane.  Blocks run one at a
 // time; dynamic LDS and the chunk partials start as NaN so that a read of an
-// entry the kernel never wrote shows in the output.  The geometry and tables
-// are built as spect_plan_create builds them (capi.hip).
+// entry the kernel never wrote shows in the output.
 //
-//   fwd_cpu <dir>
-// <dir>/meta.txt: N Nz A L kr kz n_radii n_sel
-// <dir>/f.bin float32 [N][N][Nz], angles.bin float32 [A], sel.bin int32 [n_sel],
-// ridx.bin int32 [A], taps_r.bin float32 [n_radii][P][kr], taps_z.bin [..][kz]
-// writes <dir>/proj.bin float32 [n_sel][N][Nz]; prints "nan_partial <count>".
-#include <hip/hip_runtime.h>
-
-#include <cstdio>
-#include <limits>
-#include <string>
-#include <thread>
-#include <vector>
-
+//   fwd_cpu <dir>     (problem files as in cpu_runtime.hpp, plus f.bin
+//                      float32 [N][N][Nz]); writes <dir>/proj.bin float32
+//                      [n_sel][N][Nz] and prints "H .. blocks .. threads ..
+//                      nan_partial .."
+#include "cpu_runtime.hpp"
 #include "forward_body.hpp"
 
-thread_local cpu_dim3 threadIdx;
-cpu_dim3 blockDim;
-std::barrier<> *cpu_block_barrier;
-
 using namespace spect;
-
-namespace {
-
-template <typename T> std::vector<T> read_file(const std::string &path, size_t count) {
-  std::vector<T> v(count);
-  FILE *fp = std::fopen(path.c_str(), "rb");
-  if (!fp || std::fread(v.data(), sizeof(T), count, fp) != count) {
-    std::fprintf(stderr, "cannot read %zu items from %s\n", count, path.c_str());
-    std::exit(2);
-  }
-  std::fclose(fp);
-  return v;
-}
-
-std::vector<float> host_linspace(int P) {
-  std::vector<float> v(P);
-  const float step = 2.0f / float(P - 1);
-  const int halfway = P / 2;
-  for (int k = 0; k < P; ++k) v[k] = k < halfway ? -1.0f + step * float(k) : 1.0f - step * float(P - k - 1);
-  return v;
-}
-
-std::vector<float> widen(const std::vector<float> &src, int rows, int k, int K, bool unit_if_empty) {
-  std::vector<float> out((size_t)rows * K, 0.0f);
-  const int off = (K - k) / 2;
-  for (int r = 0; r < rows; ++r) {
-    if (k > 0)
-      for (int t = 0; t < k; ++t) out[(size_t)r * K + off + t] = src[(size_t)r * k + t];
-    else if (unit_if_empty)
-      out[(size_t)r * K + K / 2] = 1.0f;
-  }
-  return out;
-}
-
-int block_threads(int Nz, int run_len, int splits) {
-  const int runs = (Nz + run_len - 1) / run_len;
-  const int nzc = (Nz + 63) / 64 * 64;
-  int t = kTJ * runs;
-  if (splits * nzc > t) t = splits * nzc;
-  return (t + 63) / 64 * 64;
-}
-
-template <int H>
-void run_blocks(const FwdJob<float> &J, int bt, unsigned char *smem, size_t smem_bytes) {
-  constexpr int RZ = FwdTile<float>::RZ, QS = FwdTile<float>::QS;
-  std::barrier<> bar(bt);
-  cpu_block_barrier = &bar;
-  const Geo<float> &g = J.g;
-  const int nblocks = J.n * g.ntiles * g.C;
-  std::vector<std::thread> lanes;
-  for (int t = 0; t < bt; ++t)
-    lanes.emplace_back([&, t] {
-      threadIdx.x = t;
-      for (int b = 0; b < nblocks; ++b) {
-        if (t == 0) {
-          float *s = reinterpret_cast<float *>(smem);
-          for (size_t q = 0; q < smem_bytes / sizeof(float); ++q) s[q] = std::numeric_limits<float>::quiet_NaN();
-        }
-        bar.arrive_and_wait();
-        // grid (n, ntiles, C) of launch_forward
-        const int al = b / (g.ntiles * g.C), jt = (b / g.C) % g.ntiles, c = b % g.C;
-        fwd_body<float, RZ, QS, H, false, false>(J, al, jt, c, smem);
-        bar.arrive_and_wait();
-      }
-    });
-  for (auto &l : lanes) l.join();
-}
-
-}  // namespace
 
 int main(int argc, char **argv) {
   if (argc != 2) {
@@ -101,90 +19,45 @@ int main(int argc, char **argv) {
     return 2;
   }
   const std::string dir = argv[1];
-  int N, Nz, A, L, kr, kz, n_radii, n;
-  FILE *fm = std::fopen((dir + "/meta.txt").c_str(), "r");
-  if (!fm || std::fscanf(fm, "%d %d %d %d %d %d %d %d", &N, &Nz, &A, &L, &kr, &kz, &n_radii, &n) != 8) {
-    std::fprintf(stderr, "bad meta.txt\n");
-    return 2;
-  }
-  std::fclose(fm);
-  // spect_plan_create: pad, P, chunks, PSF half width (even widths compiled)
-  const int pad = (int)std::ceil((std::sqrt(2.0) * N - N) / 2.0);
-  const int P = N + 2 * pad, C = (P + L - 1) / L;
-  int H = 0;
-  if (kr > 0) {
-    H = (std::max(kr, kz) / 2 + 1) / 2 * 2;
-    if (H == 0) H = 2;
-  }
-  Geo<float> g;
-  g.N = N; g.Nz = Nz; g.P = P; g.pad = pad; g.A = A; g.H = H;
-  g.jo0 = pad; g.jo1 = pad + N;
-  g.je0 = std::max(0, g.jo0 - H); g.je1 = std::min(P, g.jo1 + H);
-  g.L = L; g.C = C;
-  g.ntiles = (g.je1 - g.je0 + kTJ - 1) / kTJ;
-  g.half = float(P - 1) / 2.0f;
-  g.dx = 1.0f;
-  g.lo = float(pad) - 2.0f;
-  g.hi = float(pad + N) + 1.0f;
-
-  const auto f = read_file<float>(dir + "/f.bin", (size_t)N * N * Nz);
-  const auto angles = read_file<float>(dir + "/angles.bin", A);
-  const auto sel = read_file<int32_t>(dir + "/sel.bin", n);
-  std::vector<int32_t> ridx(A, 0);
-  std::vector<float> taps_r, taps_z;
-  if (H > 0) {
-    const int K = 2 * H + 1, rows = n_radii * P;
-    ridx = read_file<int32_t>(dir + "/ridx.bin", A);
-    taps_r = widen(read_file<float>(dir + "/taps_r.bin", (size_t)rows * kr), rows, kr, K, false);
-    std::vector<float> tz;
-    if (kz > 0) tz = read_file<float>(dir + "/taps_z.bin", (size_t)rows * kz);
-    taps_z = widen(tz, rows, kz, K, true);
-  }
-  // theta_f = -(270 - beta) in float32, radians as float32 deg2rad
-  std::vector<float2> trig(A);
-  const float pi32 = (float)M_PI;
-  for (int a = 0; a < A; ++a) {
-    const float th = -(270.0f - angles[a]);
-    const float rad = (th * pi32) / 180.0f;
-    trig[a] = make_float2((float)std::cos((double)rad), (float)std::sin((double)rad));
-  }
-  const auto lin = host_linspace(P);
+  const cpu_build::Problem p = cpu_build::load_problem(dir);
+  const Geo<float> &g = p.g;
+  const int N = p.N, Nz = p.Nz, n = p.n, H = p.H;
+  const auto f = cpu_build::read_file<float>(dir + "/f.bin", (size_t)N * N * Nz);
   const int JO = kTJ + 2 * H;
-  std::vector<float> partial((size_t)C * n * g.ntiles * JO * Nz, std::numeric_limits<float>::quiet_NaN());
-  Tables<float> tb{trig.data(), lin.data(), ridx.data(), H > 0 ? taps_r.data() : nullptr,
-                   H > 0 ? taps_z.data() : nullptr};
-  FwdJob<float> J{g, tb, f.data(), nullptr, nullptr, 0, nullptr, sel.data(), n, partial.data()};
+  std::vector<float> partial((size_t)p.C * n * g.ntiles * JO * Nz, std::numeric_limits<float>::quiet_NaN());
+  const FwdJob<float> J{g, p.tables(), f.data(), nullptr, nullptr, 0, nullptr, p.sel.data(), n, partial.data()};
 
   // launch_fwd_h: block size, phase-A coverage, dynamic LDS
   constexpr int RZ = FwdTile<float>::RZ, QS = FwdTile<float>::QS;
-  const int bt = block_threads(Nz, RZ, QS);
+  const int bt = cpu_build::block_threads(Nz, RZ, QS);
   if (bt > 512 || !phase_a_fits<float, QS>(bt, Nz)) {
     std::fprintf(stderr, "launch configuration rejected\n");
     return 2;
   }
-  blockDim.x = bt;
   const int runs = (Nz + RZ - 1) / RZ;
-  const size_t smem_bytes =
-      (2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)L * 2 * (2 * H + 1)) * sizeof(float);
-  std::vector<unsigned char> smem(smem_bytes);
+  std::vector<unsigned char> smem(
+      (2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)p.L * 2 * (2 * H + 1)) * sizeof(float));
+  auto run = [&](auto h_c) {
+    constexpr int HH = decltype(h_c)::value;
+    cpu_build::run_grid(n, g.ntiles, p.C, bt, smem, [&](int al, int jt, int c, unsigned char *s) {
+      fwd_body<float, RZ, QS, HH, false, false>(J, al, jt, c, s);
+    });
+  };
   switch (H) {
-    case 0: run_blocks<0>(J, bt, smem.data(), smem_bytes); break;
-    case 2: run_blocks<2>(J, bt, smem.data(), smem_bytes); break;
-    case 4: run_blocks<4>(J, bt, smem.data(), smem_bytes); break;
-    case 6: run_blocks<6>(J, bt, smem.data(), smem_bytes); break;
-    case 8: run_blocks<8>(J, bt, smem.data(), smem_bytes); break;
+    case 0: run(std::integral_constant<int, 0>{}); break;
+    case 2: run(std::integral_constant<int, 2>{}); break;
+    case 4: run(std::integral_constant<int, 4>{}); break;
+    case 6: run(std::integral_constant<int, 6>{}); break;
+    case 8: run(std::integral_constant<int, 8>{}); break;
     default: std::fprintf(stderr, "PSF half width %d not built here\n", H); return 2;
   }
-  size_t nan_partial = 0;
-  for (float v : partial) nan_partial += std::isnan(v) ? 1 : 0;
+  const size_t nan_partial = cpu_build::count_nan(partial);
 
   ReduceJob<float> RJ{g, partial.data(), n, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0f};
   std::vector<float> proj((size_t)n * N * Nz);
   RJ.proj = proj.data();
   for (size_t i = 0; i < proj.size(); ++i) reduce_body<float>(RJ, i);
-  FILE *fo = std::fopen((dir + "/proj.bin").c_str(), "wb");
-  std::fwrite(proj.data(), sizeof(float), proj.size(), fo);
-  std::fclose(fo);
-  std::printf("H %d blocks %d threads %d nan_partial %zu\n", H, n * g.ntiles * C, bt, nan_partial);
+  cpu_build::write_file(dir + "/proj.bin", proj);
+  std::printf("H %d blocks %d threads %d nan_partial %zu\n", H, n * g.ntiles * p.C, bt, nan_partial);
   return 0;
 }

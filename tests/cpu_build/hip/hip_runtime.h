@@ -33,6 +33,9 @@ inline unsigned long long atomicOr(unsigned long long *p, unsigned long long v) 
   return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST);
 }
 inline int __clzll(unsigned long long v) { return __builtin_clzll(v); }
+// Block-wide OR of pred: two barriers, the partial ORs double-buffered so that
+// consecutive calls never share a slot.
+int __syncthreads_or(int pred);
 inline float __expf(float v) { return std::exp(v); }
 
 struct float2 {
@@ -45,6 +48,7 @@ struct alignas(16) float4 {
   float x, y, z, w;
 };
 inline float2 make_float2(float a, float b) { return {a, b}; }
+inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
 inline double2 make_double2(double a, double b) { return {a, b}; }
 
 typedef int hipError_t;

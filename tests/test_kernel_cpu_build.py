@@ -1,13 +1,16 @@
-"""The forward projector's block bodies (csrc/forward_body.hpp) built for the
-host with g++ and a thread-per-lane stand-in for the HIP runtime
-(tests/cpu_build), under AddressSanitizer and UBSan, against the oracle.
+"""The projector's block bodies (csrc/forward_body.hpp, csrc/backward_body.hpp)
+built for the host with g++ and a thread-per-lane stand-in for the HIP
+runtime (tests/cpu_build), under AddressSanitizer and UBSan, against the
+oracle.
 
 The GPU parity tests check the compiled kernels; this checks the kernel
 source itself on the CPU: index arithmetic, lane coverage of every phase,
-barriers and the chunk reduce, with dynamic LDS and the chunk partials
-pre-filled with NaN so that a read of an entry no lane wrote would show.
-The cases include whole-tile geometries with several phase-A passes (the
-no-PSF layout of a 128^3 object), ragged z (element loads) and PSF widths."""
+barriers, the chunk reduce and the back projection's field/gather split.
+Dynamic LDS, the forward's chunk partials and the back projection's field W
+start as NaN, so a read of an entry no lane wrote -- also one taken with
+weight zero -- would show in the output.  The cases include whole-tile
+geometries with several phase-A passes (the no-PSF layout of a 128^3 object),
+ragged z (element loads) and PSF widths."""
 from __future__ import annotations
 
 import os
@@ -22,24 +25,29 @@ from oracle import spect_oracle as so
 from tests.helpers import oracle_geometry, problem_arrays, rel_l2
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SRC = os.path.join(ROOT, "tests", "cpu_build", "fwd_cpu.cpp")
 
 
 @pytest.fixture(scope="module")
-def fwd_cpu(tmp_path_factory):
+def cpu_build(tmp_path_factory):
+    """Paths of the forward (fwd_cpu) and back projection (bwd_cpu) builds."""
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
-    out = str(tmp_path_factory.mktemp("cpu_build") / "fwd_cpu")
-    cmd = [gxx, "-std=c++20", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
-           "-fno-omit-frame-pointer", "-pthread", "-Wno-attributes", "-Wno-unknown-pragmas",
-           "-I", os.path.join(ROOT, "tests", "cpu_build"), "-I", os.path.join(ROOT, "pytomography_amd", "csrc"),
-           SRC, "-o", out]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0 and "barrier" in r.stderr:
-        pytest.skip("the host compiler lacks C++20 <barrier>")
-    assert r.returncode == 0, r.stderr[-4000:]
-    return out
+    out_dir = tmp_path_factory.mktemp("cpu_build")
+    exes = {}
+    for name in ("fwd_cpu", "bwd_cpu"):
+        exe = str(out_dir / name)
+        cmd = [gxx, "-std=c++20", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+               "-fno-omit-frame-pointer", "-pthread", "-Wno-attributes", "-Wno-unknown-pragmas",
+               "-I", os.path.join(ROOT, "tests", "cpu_build"),
+               "-I", os.path.join(ROOT, "pytomography_amd", "csrc"),
+               os.path.join(ROOT, "tests", "cpu_build", name + ".cpp"), "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0 and "barrier" in r.stderr:
+            pytest.skip("the host compiler lacks C++20 <barrier>")
+        assert r.returncode == 0, r.stderr[-4000:]
+        exes[name] = exe
+    return exes
 
 
 def _psf_tables(pa):
@@ -65,8 +73,7 @@ def _psf_tables(pa):
     return args["psf_kr"], args["psf_kz"], args["n_radii"], ridx, tr32, tz32
 
 
-def _run(fwd_cpu, tmp_path, pa, sel, chunk_planes=16):
-    d = tmp_path
+def _write_problem(d, pa, sel, chunk_planes=16):
     n, nz, A = pa["n"], pa["nz"], len(pa["angles"])
     kr = kz = n_radii = 0
     if pa["psf"]:
@@ -76,15 +83,25 @@ def _run(fwd_cpu, tmp_path, pa, sel, chunk_planes=16):
         if tz is not None:
             tz.astype(np.float32).tofile(d / "taps_z.bin")
     (d / "meta.txt").write_text(f"{n} {nz} {A} {chunk_planes} {kr} {kz} {n_radii} {len(sel)}\n")
-    np.ascontiguousarray(pa["f"], dtype=np.float32).tofile(d / "f.bin")
     np.asarray(pa["angles"], dtype=np.float32).tofile(d / "angles.bin")
     np.asarray(sel, dtype=np.int32).tofile(d / "sel.bin")
+
+
+def _exec(exe, d):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    r = subprocess.run([fwd_cpu, str(d)], capture_output=True, text=True, env=env, timeout=600)
+    r = subprocess.run([exe, str(d)], capture_output=True, text=True, env=env, timeout=900)
     assert r.returncode == 0, (r.stdout + r.stderr)[-4000:]
-    stats = dict(zip(r.stdout.split()[0::2], map(int, r.stdout.split()[1::2])))
-    proj = torch.from_numpy(np.fromfile(d / "proj.bin", dtype=np.float32).reshape(len(sel), n, nz))
-    return proj, stats
+    words = r.stdout.split()
+    return dict(zip(words[0::2], map(int, words[1::2])))
+
+
+def _run(exe, tmp_path, pa, sel, chunk_planes=16):
+    """Forward of pa["f"] over the angles ``sel`` by the CPU build."""
+    _write_problem(tmp_path, pa, sel, chunk_planes)
+    np.ascontiguousarray(pa["f"], dtype=np.float32).tofile(tmp_path / "f.bin")
+    stats = _exec(exe, tmp_path)
+    proj = np.fromfile(tmp_path / "proj.bin", dtype=np.float32).reshape(len(sel), pa["n"], pa["nz"])
+    return torch.from_numpy(proj), stats
 
 
 CASES = [
@@ -97,10 +114,10 @@ CASES = [
 
 
 @pytest.mark.parametrize("name,spec,sel", CASES, ids=[c[0] for c in CASES])
-def test_forward_body_on_cpu_matches_oracle(fwd_cpu, tmp_path, name, spec, sel):
+def test_forward_body_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel):
     spec = dict(spec)
     pa = problem_arrays(spec.pop("n"), spec.pop("n_angles"), seed=5, **spec)
-    proj, stats = _run(fwd_cpu, tmp_path, pa, sel)
+    proj, stats = _run(cpu_build["fwd_cpu"], tmp_path, pa, sel)
     ref = so.forward(oracle_geometry(pa), torch.from_numpy(pa["f"]), torch.tensor(sel))
     errs = [rel_l2(proj[k], ref[k]) for k in range(len(sel))]
     print(f"{name}: H {stats['H']}, {stats['blocks']} blocks x {stats['threads']} lanes, "
@@ -112,7 +129,7 @@ def test_forward_body_on_cpu_matches_oracle(fwd_cpu, tmp_path, name, spec, sel):
 
 @pytest.mark.skipif(os.environ.get("SPECT_CPU_BUILD_FULL") != "1",
                     reason="about 10 minutes: set SPECT_CPU_BUILD_FULL=1")
-def test_forward_body_on_cpu_plain_128(fwd_cpu, tmp_path):
+def test_forward_body_on_cpu_plain_128(cpu_build, tmp_path):
     """The no-PSF, no-attenuation forward of the 128^3 ellipsoid phantom (seed
     0) over all 120 angles, the data-simulation call of the plain OSEM
     workload, run from the kernel source on the CPU."""
@@ -120,7 +137,7 @@ def test_forward_body_on_cpu_plain_128(fwd_cpu, tmp_path):
     n, A = 128, 120
     pa = dict(n=n, nz=n, angles=synthetic.angles(A), f=synthetic.ellipsoid_phantom(n, 0), mu=None, psf=False,
               radii=np.full(A, synthetic.RADIUS))
-    proj, stats = _run(fwd_cpu, tmp_path, pa, list(range(A)))
+    proj, stats = _run(cpu_build["fwd_cpu"], tmp_path, pa, list(range(A)))
     ref = so.forward(oracle_geometry(pa), torch.from_numpy(pa["f"]))
     errs = [rel_l2(proj[k], ref[k]) for k in range(A)]
     print(f"plain 128^3 x 120: {stats['blocks']} blocks x {stats['threads']} lanes, max rel-L2 "
@@ -128,3 +145,52 @@ def test_forward_body_on_cpu_plain_128(fwd_cpu, tmp_path):
     assert stats["nan_partial"] == 0
     assert bool(torch.isfinite(proj).all()) and bool((proj >= 0).all())
     assert max(errs) <= 1e-5
+
+
+@pytest.mark.parametrize("name,spec,sel", CASES, ids=[c[0] for c in CASES])
+def test_backward_bodies_on_cpu_match_oracle(cpu_build, tmp_path, name, spec, sel):
+    """Field + gather (reference back projection, mode 0) and field + exact
+    adjoint gather (mode 1) of seeded projections, against the oracle's
+    reference backward and its autograd transpose of the forward."""
+    spec = dict(spec)
+    pa = problem_arrays(spec.pop("n"), spec.pop("n_angles"), seed=6, **spec)
+    _write_problem(tmp_path, pa, sel)
+    y = torch.rand((len(sel), pa["n"], pa["nz"]), generator=torch.Generator().manual_seed(7))
+    y.numpy().astype(np.float32).tofile(tmp_path / "p.bin")
+    stats = _exec(cpu_build["bwd_cpu"], tmp_path)
+    shape = (pa["n"], pa["n"], pa["nz"])
+    ref_cpu = torch.from_numpy(np.fromfile(tmp_path / "bp_ref.bin", dtype=np.float32).reshape(shape))
+    adj_cpu = torch.from_numpy(np.fromfile(tmp_path / "bp_adj.bin", dtype=np.float32).reshape(shape))
+    g = oracle_geometry(pa)
+    ref = so.backward(g, y, torch.tensor(sel))
+    adj = so.exact_adjoint(g, y, torch.tensor(sel))
+    e_ref, e_adj = rel_l2(ref_cpu, ref), rel_l2(adj_cpu, adj)
+    print(f"{name}: H {stats['H']}, W written {stats['w_written']} of {stats['w_total']}, "
+          f"reference rel-L2 {e_ref:.1e}, adjoint rel-L2 {e_adj:.1e}")
+    assert bool(torch.isfinite(ref_cpu).all()) and bool(torch.isfinite(adj_cpu).all())
+    assert e_ref <= 1e-5 and e_adj <= 1e-5
+
+
+@pytest.mark.skipif(os.environ.get("SPECT_CPU_BUILD_FULL") != "1",
+                    reason="about 20 minutes: set SPECT_CPU_BUILD_FULL=1")
+def test_backward_bodies_on_cpu_plain_128(cpu_build, tmp_path):
+    """Reference back projection and exact adjoint of seeded 120 x 128 x 128
+    projections (no PSF, no attenuation) from the kernel source on the CPU."""
+    from pytomography_amd import synthetic
+    n, A = 128, 120
+    pa = dict(n=n, nz=n, angles=synthetic.angles(A), f=None, mu=None, psf=False,
+              radii=np.full(A, synthetic.RADIUS))
+    sel = list(range(A))
+    _write_problem(tmp_path, pa, sel)
+    y = torch.rand((A, n, n), generator=torch.Generator().manual_seed(8))
+    y.numpy().astype(np.float32).tofile(tmp_path / "p.bin")
+    stats = _exec(cpu_build["bwd_cpu"], tmp_path)
+    ref_cpu = torch.from_numpy(np.fromfile(tmp_path / "bp_ref.bin", dtype=np.float32).reshape(n, n, n))
+    adj_cpu = torch.from_numpy(np.fromfile(tmp_path / "bp_adj.bin", dtype=np.float32).reshape(n, n, n))
+    g = oracle_geometry(pa)
+    e_ref = rel_l2(ref_cpu, so.backward(g, y))
+    e_adj = rel_l2(adj_cpu, so.exact_adjoint(g, y))
+    print(f"plain 128^3 x 120 back projection: W written {stats['w_written']} of {stats['w_total']}, "
+          f"reference rel-L2 {e_ref:.1e}, adjoint rel-L2 {e_adj:.1e}")
+    assert bool(torch.isfinite(ref_cpu).all()) and bool(torch.isfinite(adj_cpu).all())
+    assert e_ref <= 1e-5 and e_adj <= 1e-5
