@@ -38,105 +38,15 @@ __global__ __launch_bounds__(1024) void att_table_kernel(Geo<R> g, Tables<R> tb,
                                                         const R *__restrict__ mu,
                                                         const int32_t *__restrict__ angle_idx,
                                                         double *__restrict__ table) {
-  __shared__ Src<R> src[kTJ];
-  const int al = blockIdx.x, jt = blockIdx.y;
-  const int ang = ld_uniform(angle_idx + al);
-  if ((unsigned)ang >= (unsigned)g.A) return;
-  const int tid = threadIdx.x;
-  const int j0 = g.je0 + jt * kTJ;
-  const int tj = min(kTJ, g.je1 - j0);
-  const int runs = (g.Nz + RZ - 1) / RZ;
-  const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
-  const bool mine = run < runs && jl < tj;
-  const int n_jext = g.je1 - g.je0;
-  const auto cs = ld_uniform(tb.trig + ang);
-  double S[RZ];
-#pragma unroll
-  for (int m = 0; m < RZ; ++m) S[m] = 0.0;
-  auto row = [&](int c, int m) -> size_t {
-    return (((size_t)al * g.C + c) * n_jext + (j0 - g.je0 + jl)) * g.Nz + z0 + m;
-  };
-  if (mine) {
-#pragma unroll
-    for (int m = 0; m < RZ; ++m)
-      if (z0 + m < g.Nz) table[row(g.C - 1, m)] = 0.0;
-  }
-  for (int i = g.P - 1; i >= 0; --i) {
-    int inside = 0;
-    if (tid < tj) {
-      R ix, iy;
-      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], ld_uniform(tb.lin + i), g.half, ix, iy);
-      src[tid] = make_src<R>(ix, iy, g.pad);
-      inside = in_footprint<R>(ix, iy, g);
-    }
-    if (__syncthreads_or(inside)) {
-      if (mine) {
-        const Taps4<R> t = taps4<R>(src[jl], g.N, g.Nz);
-        R x[RZ];
-        gather_run<R, RZ>(mu, t, z0, g.Nz, x);
-#pragma unroll
-        for (int m = 0; m < RZ; ++m) S[m] += (double)(x[m] * g.dx);
-      }
-      __syncthreads();  // src is rewritten by the next plane
-    }
-    if (mine && i > 0 && i % g.L == 0) {
-      const int c = i / g.L - 1;
-#pragma unroll
-      for (int m = 0; m < RZ; ++m)
-        if (z0 + m < g.Nz) table[row(c, m)] = S[m];
-    }
-  }
+  att_table_body<R, RZ>(g, tb, mu, angle_idx, table, blockIdx.x, blockIdx.y);
 }
 
-// The attenuation factor itself, a(i, j, z) = exp(-(S(i) - x(i)/2)) in R, for
-// every plane whose tile footprint is non-empty (the planes the projection
-// kernels visit), from the same FP64 march as att_table_kernel: the forward
-// and field kernels then read it instead of gathering mu (bitwise the same
-// factors).  field layout: [angle][P][je1-je0][Nz].
 template <typename R, int RZ>
 __global__ __launch_bounds__(1024) void att_field_kernel(Geo<R> g, Tables<R> tb,
                                                         const R *__restrict__ mu,
                                                         const int32_t *__restrict__ angle_idx,
                                                         R *__restrict__ field) {
-  __shared__ Src<R> src[kTJ];
-  const int al = blockIdx.x, jt = blockIdx.y;
-  const int ang = ld_uniform(angle_idx + al);
-  if ((unsigned)ang >= (unsigned)g.A) return;
-  const int tid = threadIdx.x;
-  const int j0 = g.je0 + jt * kTJ;
-  const int tj = min(kTJ, g.je1 - j0);
-  const int runs = (g.Nz + RZ - 1) / RZ;
-  const int jl = tid % kTJ, run = tid / kTJ, z0 = run * RZ;
-  const bool mine = run < runs && jl < tj;
-  const int n_jext = g.je1 - g.je0;
-  const auto cs = ld_uniform(tb.trig + ang);
-  double S[RZ];
-#pragma unroll
-  for (int m = 0; m < RZ; ++m) S[m] = 0.0;
-  for (int i = g.P - 1; i >= 0; --i) {
-    int inside = 0;
-    if (tid < tj) {
-      R ix, iy;
-      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + tid], ld_uniform(tb.lin + i), g.half, ix, iy);
-      src[tid] = make_src<R>(ix, iy, g.pad);
-      inside = in_footprint<R>(ix, iy, g);
-    }
-    if (__syncthreads_or(inside)) {
-      if (mine) {
-        const Taps4<R> t = taps4<R>(src[jl], g.N, g.Nz);
-        R x[RZ];
-        gather_run<R, RZ>(mu, t, z0, g.Nz, x);
-        R *dst = field + (((size_t)al * g.P + i) * n_jext + (j0 - g.je0 + jl)) * g.Nz + z0;
-#pragma unroll
-        for (int m = 0; m < RZ; ++m) {
-          const R xm = x[m] * g.dx;
-          S[m] += (double)xm;
-          if (z0 + m < g.Nz) dst[m] = att_factor<R>(S[m], xm);
-        }
-      }
-      __syncthreads();  // src is rewritten by the next plane
-    }
-  }
+  att_field_body<R, RZ>(g, tb, mu, angle_idx, field, blockIdx.x, blockIdx.y);
 }
 
 template <typename R, int RZ, int QS, int H, bool TRB, bool AF>

@@ -3,7 +3,8 @@
 // tests/cpu_build/hip, one std::thread per GPU lane for the field blocks.
 // The field W starts as NaN, as do the LDS buffers, so that a read of an entry
 // the field kernel never wrote -- also one taken with weight zero -- shows in
-// the output.
+// the output.  With attenuation the field or suffix table is made first by the
+// march bodies (cpu_attenuation.hpp).
 //
 //   bwd_cpu <dir>     (problem files as in cpu_runtime.hpp, plus p.bin
 //                      float32 [n_sel][N][Nz], the projections of the
@@ -11,8 +12,8 @@
 //                      back projection, gather mode 0) and <dir>/bp_adj.bin
 //                      (exact adjoint, mode 1), float32 [N][N][Nz], and
 //                      prints "H .. w_written .. w_total .."
-#include "cpu_runtime.hpp"
 #include "backward_body.hpp"
+#include "cpu_attenuation.hpp"
 
 using namespace spect;
 
@@ -28,8 +29,9 @@ int main(int argc, char **argv) {
   const auto proj = cpu_build::read_file<float>(dir + "/p.bin", (size_t)n * N * Nz);
   const size_t n_jext = g.je1 - g.je0;
   std::vector<float> W((size_t)n * p.P * n_jext * Nz, std::numeric_limits<float>::quiet_NaN());
-  const FieldJob<float> FJ{g, p.tables(), proj.data(), nullptr, nullptr, 0, nullptr, p.sel.data(), n,
-                           W.data(), nullptr, nullptr, nullptr};
+  const cpu_build::Attenuation att = cpu_build::build_attenuation(p);
+  const FieldJob<float> FJ{g, p.tables(), proj.data(), att.mu, att.att, att.att ? 1 : 0, att.attf,
+                           p.sel.data(), n, W.data(), nullptr, nullptr, nullptr};
 
   // launch_field_h: block size and dynamic LDS
   constexpr int RZ = BpTile<float>::RZ, QS = BpTile<float>::QS;
@@ -44,7 +46,8 @@ int main(int argc, char **argv) {
   auto run = [&](auto h_c) {
     constexpr int HH = decltype(h_c)::value;
     cpu_build::run_grid(n, g.ntiles, p.C, bt, smem, [&](int al, int jt, int c, unsigned char *s) {
-      field_body<float, RZ, QS, HH, false, false>(FJ, al, jt, c, s);
+      if (att.attf) field_body<float, RZ, QS, HH, false, true>(FJ, al, jt, c, s);
+      else field_body<float, RZ, QS, HH, false, false>(FJ, al, jt, c, s);
     });
   };
   switch (H) {

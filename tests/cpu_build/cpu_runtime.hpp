@@ -4,9 +4,11 @@
 // linspace tables, PSF taps widened to K = 2H+1.  Included once by each CPU
 // build executable.
 //
-// Problem directory: meta.txt "N Nz A L kr kz n_radii n_sel", angles.bin
-// float32 [A], sel.bin int32 [n_sel], and with a PSF ridx.bin int32 [A],
-// taps_r.bin float32 [n_radii][P][kr], taps_z.bin float32 [n_radii][P][kz].
+// Problem directory: meta.txt "N Nz A L kr kz n_radii n_sel att_mode dx"
+// (att_mode 0: none, 1: attenuation field, 2: FP64 suffix table; dx in cm),
+// angles.bin float32 [A], sel.bin int32 [n_sel]; with a PSF ridx.bin int32
+// [A], taps_r.bin float32 [n_radii][P][kr], taps_z.bin float32
+// [n_radii][P][kz]; with attenuation mu.bin float32 [N][N][Nz].
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -71,10 +73,11 @@ inline int block_threads(int Nz, int run_len, int splits) {
 }
 
 struct Problem {
-  int N, Nz, A, L, kr, kz, n_radii, n, pad, P, C, H;
+  int N, Nz, A, L, kr, kz, n_radii, n, att_mode, pad, P, C, H;
+  float dx;
   spect::Geo<float> g;
   std::vector<float2> trig;
-  std::vector<float> lin, taps_r, taps_z;
+  std::vector<float> lin, taps_r, taps_z, mu;
   std::vector<int32_t> ridx, sel;
   spect::Tables<float> tables() const {
     return {trig.data(), lin.data(), ridx.data(), H > 0 ? taps_r.data() : nullptr,
@@ -97,8 +100,8 @@ inline std::vector<float> widen(const std::vector<float> &src, int rows, int k, 
 inline Problem load_problem(const std::string &dir) {
   Problem p;
   FILE *fm = std::fopen((dir + "/meta.txt").c_str(), "r");
-  if (!fm || std::fscanf(fm, "%d %d %d %d %d %d %d %d", &p.N, &p.Nz, &p.A, &p.L, &p.kr, &p.kz,
-                         &p.n_radii, &p.n) != 8) {
+  if (!fm || std::fscanf(fm, "%d %d %d %d %d %d %d %d %d %f", &p.N, &p.Nz, &p.A, &p.L, &p.kr, &p.kz,
+                         &p.n_radii, &p.n, &p.att_mode, &p.dx) != 10) {
     std::fprintf(stderr, "bad meta.txt\n");
     std::exit(2);
   }
@@ -118,10 +121,11 @@ inline Problem load_problem(const std::string &dir) {
   g.L = p.L; g.C = p.C;
   g.ntiles = (g.je1 - g.je0 + spect::kTJ - 1) / spect::kTJ;
   g.half = float(p.P - 1) / 2.0f;
-  g.dx = 1.0f;
+  g.dx = p.dx;
   g.lo = float(p.pad) - 2.0f;
   g.hi = float(p.pad + p.N) + 1.0f;
   const auto angles = read_file<float>(dir + "/angles.bin", p.A);
+  if (p.att_mode != 0) p.mu = read_file<float>(dir + "/mu.bin", (size_t)p.N * p.N * p.Nz);
   p.sel = read_file<int32_t>(dir + "/sel.bin", p.n);
   p.ridx.assign(p.A, 0);
   if (p.H > 0) {
@@ -156,6 +160,8 @@ void run_grid(int n, int ntiles, int C, int bt, std::vector<unsigned char> &smem
   std::barrier<> bar(bt);
   cpu_block_barrier = &bar;
   blockDim.x = bt;
+  g_or_slot[0].store(0);  // fresh lanes start their __syncthreads_or count at 0
+  g_or_slot[1].store(0);
   const int nblocks = n * ntiles * C;
   std::vector<std::thread> lanes;
   for (int t = 0; t < bt; ++t)

@@ -4,12 +4,14 @@
 // time; dynamic LDS and the chunk partials start as NaN so that a read of an
 // entry the kernel never wrote shows in the output.
 //
+// With attenuation the field or suffix table is made first by the march
+// bodies (cpu_attenuation.hpp) and the body runs in the matching mode.
+//
 //   fwd_cpu <dir>     (problem files as in cpu_runtime.hpp, plus f.bin
 //                      float32 [N][N][Nz]); writes <dir>/proj.bin float32
 //                      [n_sel][N][Nz] and prints "H .. blocks .. threads ..
 //                      nan_partial .."
-#include "cpu_runtime.hpp"
-#include "forward_body.hpp"
+#include "cpu_attenuation.hpp"
 
 using namespace spect;
 
@@ -25,7 +27,9 @@ int main(int argc, char **argv) {
   const auto f = cpu_build::read_file<float>(dir + "/f.bin", (size_t)N * N * Nz);
   const int JO = kTJ + 2 * H;
   std::vector<float> partial((size_t)p.C * n * g.ntiles * JO * Nz, std::numeric_limits<float>::quiet_NaN());
-  const FwdJob<float> J{g, p.tables(), f.data(), nullptr, nullptr, 0, nullptr, p.sel.data(), n, partial.data()};
+  const cpu_build::Attenuation att = cpu_build::build_attenuation(p);
+  const FwdJob<float> J{g, p.tables(), f.data(), att.mu, att.att, att.att ? 1 : 0, att.attf, p.sel.data(), n,
+                        partial.data()};
 
   // launch_fwd_h: block size, phase-A coverage, dynamic LDS
   constexpr int RZ = FwdTile<float>::RZ, QS = FwdTile<float>::QS;
@@ -40,7 +44,8 @@ int main(int argc, char **argv) {
   auto run = [&](auto h_c) {
     constexpr int HH = decltype(h_c)::value;
     cpu_build::run_grid(n, g.ntiles, p.C, bt, smem, [&](int al, int jt, int c, unsigned char *s) {
-      fwd_body<float, RZ, QS, HH, false, false>(J, al, jt, c, s);
+      if (att.attf) fwd_body<float, RZ, QS, HH, false, true>(J, al, jt, c, s);
+      else fwd_body<float, RZ, QS, HH, false, false>(J, al, jt, c, s);
     });
   };
   switch (H) {

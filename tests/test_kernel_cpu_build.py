@@ -73,7 +73,8 @@ def _psf_tables(pa):
     return args["psf_kr"], args["psf_kz"], args["n_radii"], ridx, tr32, tz32
 
 
-def _write_problem(d, pa, sel, chunk_planes=16):
+def _write_problem(d, pa, sel, chunk_planes=16, att_mode=0):
+    """att_mode 0: no attenuation, 1: attenuation field, 2: FP64 suffix table."""
     n, nz, A = pa["n"], pa["nz"], len(pa["angles"])
     kr = kz = n_radii = 0
     if pa["psf"]:
@@ -82,7 +83,11 @@ def _write_problem(d, pa, sel, chunk_planes=16):
         tr.astype(np.float32).tofile(d / "taps_r.bin")
         if tz is not None:
             tz.astype(np.float32).tofile(d / "taps_z.bin")
-    (d / "meta.txt").write_text(f"{n} {nz} {A} {chunk_planes} {kr} {kz} {n_radii} {len(sel)}\n")
+    if att_mode:
+        np.ascontiguousarray(pa["mu"], dtype=np.float32).tofile(d / "mu.bin")
+    from pytomography_amd import synthetic
+    (d / "meta.txt").write_text(f"{n} {nz} {A} {chunk_planes} {kr} {kz} {n_radii} {len(sel)} {att_mode} "
+                                f"{synthetic.DX}\n")
     np.asarray(pa["angles"], dtype=np.float32).tofile(d / "angles.bin")
     np.asarray(sel, dtype=np.int32).tofile(d / "sel.bin")
 
@@ -95,9 +100,9 @@ def _exec(exe, d):
     return dict(zip(words[0::2], map(int, words[1::2])))
 
 
-def _run(exe, tmp_path, pa, sel, chunk_planes=16):
+def _run(exe, tmp_path, pa, sel, chunk_planes=16, att_mode=0):
     """Forward of pa["f"] over the angles ``sel`` by the CPU build."""
-    _write_problem(tmp_path, pa, sel, chunk_planes)
+    _write_problem(tmp_path, pa, sel, chunk_planes, att_mode)
     np.ascontiguousarray(pa["f"], dtype=np.float32).tofile(tmp_path / "f.bin")
     stats = _exec(exe, tmp_path)
     proj = np.fromfile(tmp_path / "proj.bin", dtype=np.float32).reshape(len(sel), pa["n"], pa["nz"])
@@ -105,19 +110,24 @@ def _run(exe, tmp_path, pa, sel, chunk_planes=16):
 
 
 CASES = [
+    # name, problem, angles, attenuation mode (1: field, 2: FP64 suffix table)
     # whole 32-column tiles, 4 phase-A passes per lane: the no-PSF layout of 128^3
-    ("plain_whole_tiles", dict(n=64, n_angles=60), [0, 7, 15, 22, 30, 37, 45, 59]),
-    ("plain_ragged", dict(n=21, n_angles=9, nz=13), list(range(9))),
-    ("psf", dict(n=24, n_angles=12, psf=True), [0, 3, 5, 11]),
-    ("psf_ragged", dict(n=21, n_angles=9, psf=True, nz=13), [1, 4, 8]),
+    ("plain_whole_tiles", dict(n=64, n_angles=60), [0, 7, 15, 22, 30, 37, 45, 59], 0),
+    ("plain_ragged", dict(n=21, n_angles=9, nz=13), list(range(9)), 0),
+    ("psf", dict(n=24, n_angles=12, psf=True), [0, 3, 5, 11], 0),
+    ("psf_ragged", dict(n=21, n_angles=9, psf=True, nz=13), [1, 4, 8], 0),
+    ("att_field_psf", dict(n=24, n_angles=12, att=True, psf=True), [0, 2, 7, 10], 1),
+    ("att_table_psf", dict(n=24, n_angles=12, att=True, psf=True), [1, 5, 11], 2),
+    ("att_field_ragged", dict(n=21, n_angles=9, att=True, psf=True, nz=13), [0, 3, 6], 1),
+    ("att_table_whole_tiles", dict(n=32, n_angles=16, att=True), [0, 4, 9, 15], 2),
 ]
 
 
-@pytest.mark.parametrize("name,spec,sel", CASES, ids=[c[0] for c in CASES])
-def test_forward_body_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel):
+@pytest.mark.parametrize("name,spec,sel,att_mode", CASES, ids=[c[0] for c in CASES])
+def test_forward_body_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel, att_mode):
     spec = dict(spec)
     pa = problem_arrays(spec.pop("n"), spec.pop("n_angles"), seed=5, **spec)
-    proj, stats = _run(cpu_build["fwd_cpu"], tmp_path, pa, sel)
+    proj, stats = _run(cpu_build["fwd_cpu"], tmp_path, pa, sel, att_mode=att_mode)
     ref = so.forward(oracle_geometry(pa), torch.from_numpy(pa["f"]), torch.tensor(sel))
     errs = [rel_l2(proj[k], ref[k]) for k in range(len(sel))]
     print(f"{name}: H {stats['H']}, {stats['blocks']} blocks x {stats['threads']} lanes, "
@@ -147,14 +157,14 @@ def test_forward_body_on_cpu_plain_128(cpu_build, tmp_path):
     assert max(errs) <= 1e-5
 
 
-@pytest.mark.parametrize("name,spec,sel", CASES, ids=[c[0] for c in CASES])
-def test_backward_bodies_on_cpu_match_oracle(cpu_build, tmp_path, name, spec, sel):
+@pytest.mark.parametrize("name,spec,sel,att_mode", CASES, ids=[c[0] for c in CASES])
+def test_backward_bodies_on_cpu_match_oracle(cpu_build, tmp_path, name, spec, sel, att_mode):
     """Field + gather (reference back projection, mode 0) and field + exact
     adjoint gather (mode 1) of seeded projections, against the oracle's
     reference backward and its autograd transpose of the forward."""
     spec = dict(spec)
     pa = problem_arrays(spec.pop("n"), spec.pop("n_angles"), seed=6, **spec)
-    _write_problem(tmp_path, pa, sel)
+    _write_problem(tmp_path, pa, sel, att_mode=att_mode)
     y = torch.rand((len(sel), pa["n"], pa["nz"]), generator=torch.Generator().manual_seed(7))
     y.numpy().astype(np.float32).tofile(tmp_path / "p.bin")
     stats = _exec(cpu_build["bwd_cpu"], tmp_path)
