@@ -52,7 +52,7 @@ __global__ __launch_bounds__(1024) void att_field_kernel(Geo<R> g, Tables<R> tb,
 template <typename R, int RZ, int QS, int H, bool TRB, bool AF>
 __global__ __launch_bounds__(512) void fwd_kernel(FwdJob<R> J) {
   extern __shared__ __align__(16) unsigned char smem[];
-  fwd_body<R, RZ, QS, H, TRB, AF>(J, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+  fwd_body<R, RZ, QS, H, TRB, AF>(J, blockIdx.x, blockIdx.y, chunk_centre_out(blockIdx.z, J.g.C), smem);
 }
 
 // Batched: block b belongs to job find_job(b); within a job the blocks run
