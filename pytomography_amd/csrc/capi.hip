@@ -20,6 +20,8 @@ struct spect_plan {
   double dx, dy, dz;
   int H;                 // PSF half width used by the kernels (0: none)
   int L, C;              // planes per chunk, chunk count
+  int L_field;           // planes per chunk of the back projection's field kernel in
+                         // attenuation-field mode (no state carried across chunks)
   int n_radii;
   bool has64;
   int device;
@@ -238,7 +240,15 @@ int backward_impl(const spect_plan *p, int mode, int out_mode, const R *proj, co
   int rc = att_rows<R>(p, g, mu, table, field, idx, n, ws, l, st, &att, &global, &attf);
   if (rc) return rc;
   R *W = reinterpret_cast<R *>(ws + l.w_off);
-  HIP_TRY(launch_bp_field<R>(g, tables<R>(p), proj, mu, att, global, attf, idx, n, W, nullptr,
+  // With the attenuation field the field kernel reads per-plane factors and
+  // carries nothing from one chunk to the next, so it runs with its own chunk
+  // length; the suffix-table mode reads rows at the plan's chunk boundaries.
+  Geo<R> gf = g;
+  if (attf) {
+    gf.L = p->L_field;
+    gf.C = (p->P + gf.L - 1) / gf.L;
+  }
+  HIP_TRY(launch_bp_field<R>(gf, tables<R>(p), proj, mu, att, global, attf, idx, n, W, nullptr,
                              nullptr, nullptr, st));
   HIP_TRY(launch_bp_gather<R>(g, tables<R>(p), W, idx, n, mode, out_mode, out, f, nrm, R(delta),
                               R(floor_value), nullptr, nullptr, nullptr, st));
@@ -540,6 +550,9 @@ int spect_plan_create(const spect_geom *g, spect_plan **out) {
   p->H = H;
   p->L = g->chunk_planes > 0 ? g->chunk_planes : 16;
   p->C = (p->P + p->L - 1) / p->L;
+  // Default: 24-plane field chunks (fewer blocks, each keeping its projection
+  // window for more planes); an explicit chunk_planes applies to every kernel.
+  p->L_field = g->chunk_planes > 0 ? g->chunk_planes : 24;
   p->n_radii = g->psf_kr > 0 ? g->n_radii : 1;
   p->has64 = g->angles_deg64 != nullptr && (g->psf_kr == 0 || g->taps_r64 != nullptr);
   (void)hipGetDevice(&p->device);
