@@ -56,7 +56,8 @@ __global__ __launch_bounds__(512) void fwd_kernel(FwdJob<R> J) {
 }
 
 // Batched: block b belongs to job find_job(b); within a job the blocks run
-// over (angle, tile, chunk) with the chunk fastest.
+// over (angle, tile) fastest and then over the chunks centre-out, as in the
+// single-problem grid.
 template <typename R, int RZ, int QS, int H, bool AF>
 __global__ __launch_bounds__(512) void fwd_batch_kernel(const FwdJob<R> *__restrict__ jobs,
                                                         const int *__restrict__ starts, int njobs) {
@@ -65,8 +66,9 @@ __global__ __launch_bounds__(512) void fwd_batch_kernel(const FwdJob<R> *__restr
   const int j = find_job(starts, njobs, b);
   const FwdJob<R> J = ld_uniform_struct(jobs + j);
   const int local = b - ld_uniform(starts + j);
-  const int c = local % J.g.C, r = local / J.g.C;
-  fwd_body<R, RZ, QS, H, false, AF>(J, r / J.g.ntiles, r % J.g.ntiles, c, smem);
+  const int per_chunk = J.n * J.g.ntiles;
+  const int c = chunk_centre_out(local / per_chunk, J.g.C), r = local % per_chunk;
+  fwd_body<R, RZ, QS, H, false, AF>(J, r % J.n, r / J.n, c, smem);
 }
 
 template <typename R>
