@@ -327,6 +327,13 @@ int batch_build(spect_batch *b, const spect_batch_problem *probs, int n, double 
     }
     members[k].push_back(i);
   }
+  // Within a group the largest problems come first in the block order, so a
+  // launch ends on the blocks of the small ones (results do not depend on it).
+  for (auto &m : members)
+    std::stable_sort(m.begin(), m.end(), [&](int a, int c) {
+      const spect_plan *pa = probs[a].plan, *pc = probs[c].plan;
+      return (double)pa->N * pa->N * pa->Nz * pa->A > (double)pc->N * pc->N * pc->Nz * pc->A;
+    });
   const int S = b->n_subsets;
   std::vector<unsigned char> host;
   auto put = [&](const void *src, size_t bytes) {
