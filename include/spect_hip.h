@@ -85,7 +85,9 @@ typedef struct spect_geom {
   const float *taps_z;     /* host [n_radii][P][psf_kz] */
   const double *taps_r64;  /* float64 copies (NULL: no float64 PSF) */
   const double *taps_z64;
-  int32_t chunk_planes;    /* planes per work chunk (0: default 16) */
+  int32_t chunk_planes;    /* planes per work chunk (0: default, 16 planes, and 24 for the
+                              back projection's field kernel in attenuation-field mode;
+                              a positive value applies to every kernel) */
 } spect_geom;
 
 const char *spect_last_error(void);

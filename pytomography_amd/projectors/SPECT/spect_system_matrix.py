@@ -69,7 +69,9 @@ class SPECTSystemMatrix(SystemMatrix):
         proj_meta: ``SPECTProjMeta``.
         object_initial_based_on_camera_path: not supported (must be False).
         sensitivity_dtype: dtype of the H^T 1 computation (float64 default).
-        chunk_planes: planes per kernel work unit (0: library default).
+        chunk_planes: planes per kernel work unit (0: library default -- 16, and
+            24 for the back projection's field kernel with the attenuation field;
+            a positive value applies to every kernel).
         attenuation_field: keep the attenuation factor of every rotated-frame
             voxel of every angle resident in device memory (built once per
             attenuation map) instead of resampling mu in every projection.
