@@ -1,9 +1,10 @@
-"""GPU parity at the full headline size: 128^3 object, 120 angles, attenuation
-(water body, two bone ellipsoids) and the depth-dependent PSF (K = 29), one
-15-angle subset of 8 -- the launches the OSEM benchmark times -- against the
-CPU oracle: forward projection, reference back projection, and one fused OSEM
-sub-iteration (forward, Poisson ratio with the additive term, back projection
-and update) from a non-uniform iterate.
+"""GPU parity at full size against the CPU oracle: the 128^3 object with 120
+angles, attenuation (water body, two bone ellipsoids) and the depth-dependent
+PSF (K = 29), one 15-angle subset of 8 -- the launches the OSEM benchmark
+times -- and the 192^3 object with 180 angles (K = 37, the unrolled BSREM
+geometry), one 9-angle subset of 20: forward projection, reference back
+projection, and one fused OSEM sub-iteration (forward, Poisson ratio with the
+additive term, back projection and update) from a non-uniform iterate.
 
 Bar (FP32): relative L2 <= 1e-5 per projection and for the back projection and
 the updated iterate; each result also reports its error against the float64
@@ -19,40 +20,43 @@ from tests.helpers import oracle_geometry, problem_arrays, product_system_matrix
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
-SUBSET = 3  # of 8 interleaved subsets: angles 3, 11, ..., 115
+# (side, angles, subsets, subset tested)
+SIZES = [(128, 120, 8, 3), (192, 180, 20, 7)]
 
 
-@pytest.fixture(scope="module")
-def problem():
+@pytest.fixture(scope="module", params=SIZES, ids=lambda p: f"{p[0]}^3x{p[1]}")
+def problem(request):
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    pa = problem_arrays(128, 120, seed=0, att=True, psf=True)
+    n, na, n_subsets, subset = request.param
+    pa = problem_arrays(n, na, seed=0, att=True, psf=True)
     sm = product_system_matrix(pa)
-    sm.set_n_subsets(8)
+    sm.set_n_subsets(n_subsets)
     g32 = oracle_geometry(pa, torch.float32)
     g64 = oracle_geometry(pa, torch.float64)
-    sidx = g32.subsets(8)[SUBSET]
-    return pa, sm, g32, g64, sidx
+    sidx = g32.subsets(n_subsets)[subset]
+    return pa, sm, g32, g64, sidx, n_subsets, subset
 
 
 def test_headline_subset_forward_and_back_projection(problem):
-    pa, sm, g32, g64, sidx = problem
+    pa, sm, g32, g64, sidx, _, subset = problem
     dev = torch.device("cuda")
     f = torch.from_numpy(pa["f"])
     assert sm._att_table(dev, torch.float32).att_field  # the benchmark's attenuation mode
-    out = sm.forward(f.to(dev), SUBSET).cpu()
+    tag = f"{pa['n']}^3 x {len(pa['angles'])} att+PSF (H {sm._plan(dev).psf_half}) subset {subset}"
+    out = sm.forward(f.to(dev), subset).cpu()
     ref = so.forward(g32, f, sidx)
     ref64 = so.forward(g64, f.double(), sidx)
     errs = [rel_l2(out[k], ref[k]) for k in range(len(sidx))]
-    print(f"128^3 x 120 att+PSF subset {SUBSET}: forward rel-L2 max {max(errs):.2e} "
+    print(f"{tag}: forward rel-L2 max {max(errs):.2e} "
           f"(vs fp64 {rel_l2(out, ref64):.2e})")
     assert max(errs) <= TOL
     y = torch.rand(out.shape, generator=torch.Generator().manual_seed(11))
-    b = sm.backward(y.to(dev), SUBSET).cpu()
+    b = sm.backward(y.to(dev), subset).cpu()
     rb = so.backward(g32, y, sidx)
     rb64 = so.backward(g64, y.double(), sidx)
     e = rel_l2(b, rb)
-    print(f"128^3 x 120 att+PSF subset {SUBSET}: reference back projection rel-L2 {e:.2e} "
+    print(f"{tag}: reference back projection rel-L2 {e:.2e} "
           f"(vs fp64 {rel_l2(b, rb64):.2e})")
     assert e <= TOL
 
@@ -62,21 +66,22 @@ def test_headline_osem_subiteration(problem):
     n = H^T 1 of the subset (preconditioned_gradient_ascent.py:121-122,329)."""
     from pytomography_amd.algorithms import OSEM
     from pytomography_amd.likelihoods import PoissonLogLikelihood
-    pa, sm, g32, g64, sidx = problem
+    pa, sm, g32, g64, sidx, n_subsets, subset = problem
     dev = torch.device("cuda")
     gen = torch.Generator().manual_seed(12)
     f = torch.from_numpy(pa["f"])
-    expected = so.forward(g32, f)
-    s = torch.full_like(expected, 0.05 * float(expected.mean()))
+    expected = torch.zeros((len(pa["angles"]), pa["n"], pa["nz"]))
+    expected[sidx] = so.forward(g32, f, sidx)  # only this subset's data is read
+    s = torch.full_like(expected, 0.05 * float(expected[sidx].mean()))
     data = torch.poisson(expected * 50 + s, generator=gen)
     x0 = 0.5 + torch.rand(f.shape, generator=gen)
     lk = PoissonLogLikelihood(sm, data.to(dev), additive_term=s.to(dev))
-    out = OSEM(lk, object_initial=x0.to(dev))(1, 8, n_subset_specific=SUBSET).cpu()
+    out = OSEM(lk, object_initial=x0.to(dev))(1, n_subsets, n_subset_specific=subset).cpu()
     norm = so.sensitivity(g32, sidx)
     pp = so.forward(g32, x0, sidx)
     ratio = data[sidx] / (pp + s[sidx] + so.DELTA)
     ref = x0 + x0 * (1 / (norm + so.DELTA)) * (so.backward(g32, ratio, sidx) - norm)
     ref[ref <= 0] = 0
     e = rel_l2(out, ref)
-    print(f"128^3 x 120 att+PSF subset {SUBSET}: OSEM sub-iteration rel-L2 {e:.2e}")
+    print(f"{pa['n']}^3 x {len(pa['angles'])} subset {subset}: OSEM sub-iteration rel-L2 {e:.2e}")
     assert e <= TOL
