@@ -12,14 +12,92 @@
 //                      back projection, gather mode 0) and <dir>/bp_adj.bin
 //                      (exact adjoint, mode 1), float32 [N][N][Nz], and
 //                      prints "H .. w_written .. w_total .."
+//   bwd_cpu <dir> grad  (plus f.bin and mu.bin float32 [N][N][Nz]): the
+//                      differentiable projector's VJP of u = p.bin as
+//                      spect_forward_vjp runs it -- FP64 attenuation rows of the
+//                      call's angles, the field with the mu-gradient parts
+//                      (GRAD), the prefix of chunk totals, gather mode 2 --
+//                      writing <dir>/grad_f.bin and <dir>/grad_mu.bin
 #include "backward_body.hpp"
 #include "cpu_attenuation.hpp"
 
 using namespace spect;
 
+namespace {
+
+// grad_impl (capi.hip) with the mu gradient requested.
+int run_grad(const std::string &dir, const cpu_build::Problem &p, const std::vector<float> &u,
+             std::vector<float> &W) {
+  const Geo<float> &g = p.g;
+  const int N = p.N, Nz = p.Nz, n = p.n, H = p.H, C = p.C;
+  if (p.mu.empty()) {
+    std::fprintf(stderr, "grad needs mu.bin (att_mode > 0)\n");
+    return 2;
+  }
+  const auto f = cpu_build::read_file<float>(dir + "/f.bin", (size_t)N * N * Nz);
+  const Tables<float> tb = p.tables();
+  const size_t n_jext = g.je1 - g.je0, plane = n_jext * Nz;
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  // FP64 suffix rows of the call's angles (launch_att_table over grid (n, ntiles))
+  std::vector<double> rows((size_t)n * C * plane, nan);
+  {
+    constexpr int RZ = FwdTile<float>::RZ;
+    const int bt = (kTJ * ((Nz + RZ - 1) / RZ) + 63) / 64 * 64;
+    std::vector<unsigned char> none(16);
+    cpu_build::run_grid(n, g.ntiles, 1, bt, none, [&](int al, int jt, int, unsigned char *) {
+      att_table_body<float, RZ>(g, tb, p.mu.data(), p.sel.data(), rows.data(), al, jt);
+    });
+  }
+  std::vector<double> Wm((size_t)n * p.P * plane, nan), Dsum((size_t)n * C * plane, nan);
+  const FieldJob<float> FJ{g, tb, u.data(), p.mu.data(), rows.data(), 0, nullptr, p.sel.data(), n,
+                           W.data(), f.data(), Wm.data(), Dsum.data()};
+  constexpr int RZ = BpTile<float>::RZ, QS = BpTile<float>::QS;
+  const int bt = cpu_build::block_threads(Nz, RZ, QS);
+  const int runs = (Nz + RZ - 1) / RZ;
+  std::vector<unsigned char> smem(
+      ((size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)p.L * 2 * (2 * H + 1)) * sizeof(float));
+  auto run = [&](auto h_c) {
+    constexpr int HH = decltype(h_c)::value;
+    cpu_build::run_grid(n, g.ntiles, C, bt, smem, [&](int al, int jt, int c, unsigned char *s) {
+      field_body<float, RZ, QS, HH, true, false>(FJ, al, jt, c, s);
+    });
+  };
+  switch (H) {
+    case 0: run(std::integral_constant<int, 0>{}); break;
+    case 2: run(std::integral_constant<int, 2>{}); break;
+    case 4: run(std::integral_constant<int, 4>{}); break;
+    case 6: run(std::integral_constant<int, 6>{}); break;
+    case 8: run(std::integral_constant<int, 8>{}); break;
+    default: std::fprintf(stderr, "PSF half width %d not built here\n", H); return 2;
+  }
+  // chunk_prefix_kernel: Dsum[a][c] <- sum over c' <= c
+  for (int a = 0; a < n; ++a)
+    for (size_t r = 0; r < plane; ++r) {
+      double acc = 0.0;
+      for (int c = 0; c < C; ++c) {
+        double &v = Dsum[((size_t)a * C + c) * plane + r];
+        acc += v;
+        v = acc;
+      }
+    }
+  constexpr int ZV = 4;
+  std::vector<float> grad_f((size_t)N * N * Nz, std::numeric_limits<float>::quiet_NaN());
+  std::vector<float> grad_mu(grad_f.size(), std::numeric_limits<float>::quiet_NaN());
+  const GatherJob<float> GJ{g, tb, W.data(), p.sel.data(), n, SPECT_OUT_STORE, grad_f.data(), nullptr,
+                            nullptr, 0.0f, 0.0f, Wm.data(), Dsum.data(), grad_mu.data()};
+  const size_t lanes = (size_t)N * N * ((Nz + ZV - 1) / ZV);
+  for (size_t gid = 0; gid < lanes; ++gid) gather_body<float, ZV, 2>(GJ, gid);
+  cpu_build::write_file(dir + "/grad_f.bin", grad_f);
+  cpu_build::write_file(dir + "/grad_mu.bin", grad_mu);
+  std::printf("H %d grad 1\n", H);
+  return 0;
+}
+
+}  // namespace
+
 int main(int argc, char **argv) {
-  if (argc != 2) {
-    std::fprintf(stderr, "usage: bwd_cpu <dir>\n");
+  if (argc < 2 || argc > 3 || (argc == 3 && std::string(argv[2]) != "grad")) {
+    std::fprintf(stderr, "usage: bwd_cpu <dir> [grad]\n");
     return 2;
   }
   const std::string dir = argv[1];
@@ -29,6 +107,7 @@ int main(int argc, char **argv) {
   const auto proj = cpu_build::read_file<float>(dir + "/p.bin", (size_t)n * N * Nz);
   const size_t n_jext = g.je1 - g.je0;
   std::vector<float> W((size_t)n * p.P * n_jext * Nz, std::numeric_limits<float>::quiet_NaN());
+  if (argc == 3) return run_grad(dir, p, proj, W);
   const cpu_build::Attenuation att = cpu_build::build_attenuation(p);
   const FieldJob<float> FJ{g, p.tables(), proj.data(), att.mu, att.att, att.att ? 1 : 0, att.attf,
                            p.sel.data(), n, W.data(), nullptr, nullptr, nullptr};

@@ -7,20 +7,23 @@
 // With attenuation the field or suffix table is made first by the march
 // bodies (cpu_attenuation.hpp) and the body runs in the matching mode.
 //
-//   fwd_cpu <dir>     (problem files as in cpu_runtime.hpp, plus f.bin
-//                      float32 [N][N][Nz]); writes <dir>/proj.bin float32
-//                      [n_sel][N][Nz] and prints "H .. blocks .. threads ..
-//                      nan_partial .."
+//   fwd_cpu <dir> [trb]  (problem files as in cpu_runtime.hpp, plus f.bin
+//                         float32 [N][N][Nz]); writes <dir>/proj.bin float32
+//                         [n_sel][N][Nz] and prints "H .. blocks .. threads ..
+//                         nan_partial ..".  With "trb" the body runs its
+//                         B^T mode: the transpose of the reference back
+//                         projection's gather replaces the forward gather.
 #include "cpu_attenuation.hpp"
 
 using namespace spect;
 
 int main(int argc, char **argv) {
-  if (argc != 2) {
-    std::fprintf(stderr, "usage: fwd_cpu <dir>\n");
+  if (argc < 2 || argc > 3 || (argc == 3 && std::string(argv[2]) != "trb")) {
+    std::fprintf(stderr, "usage: fwd_cpu <dir> [trb]\n");
     return 2;
   }
   const std::string dir = argv[1];
+  const bool trb = argc == 3;
   const cpu_build::Problem p = cpu_build::load_problem(dir);
   const Geo<float> &g = p.g;
   const int N = p.N, Nz = p.Nz, n = p.n, H = p.H;
@@ -44,8 +47,13 @@ int main(int argc, char **argv) {
   auto run = [&](auto h_c) {
     constexpr int HH = decltype(h_c)::value;
     cpu_build::run_grid(n, g.ntiles, p.C, bt, smem, [&](int al, int jt, int c, unsigned char *s) {
-      if (att.attf) fwd_body<float, RZ, QS, HH, false, true>(J, al, jt, c, s);
-      else fwd_body<float, RZ, QS, HH, false, false>(J, al, jt, c, s);
+      if (trb) {
+        if (att.attf) fwd_body<float, RZ, QS, HH, true, true>(J, al, jt, c, s);
+        else fwd_body<float, RZ, QS, HH, true, false>(J, al, jt, c, s);
+      } else {
+        if (att.attf) fwd_body<float, RZ, QS, HH, false, true>(J, al, jt, c, s);
+        else fwd_body<float, RZ, QS, HH, false, false>(J, al, jt, c, s);
+      }
     });
   };
   switch (H) {

@@ -92,19 +92,20 @@ def _write_problem(d, pa, sel, chunk_planes=16, att_mode=0):
     np.asarray(sel, dtype=np.int32).tofile(d / "sel.bin")
 
 
-def _exec(exe, d):
+def _exec(exe, d, extra=()):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    r = subprocess.run([exe, str(d)], capture_output=True, text=True, env=env, timeout=900)
+    r = subprocess.run([exe, str(d), *extra], capture_output=True, text=True, env=env, timeout=900)
     assert r.returncode == 0, (r.stdout + r.stderr)[-4000:]
     words = r.stdout.split()
     return dict(zip(words[0::2], map(int, words[1::2])))
 
 
-def _run(exe, tmp_path, pa, sel, chunk_planes=16, att_mode=0):
-    """Forward of pa["f"] over the angles ``sel`` by the CPU build."""
+def _run(exe, tmp_path, pa, sel, chunk_planes=16, att_mode=0, trb=False, obj=None):
+    """Forward of pa["f"] (or ``obj``) over the angles ``sel`` by the CPU
+    build; with ``trb`` the B^T mode of the same body."""
     _write_problem(tmp_path, pa, sel, chunk_planes, att_mode)
-    np.ascontiguousarray(pa["f"], dtype=np.float32).tofile(tmp_path / "f.bin")
-    stats = _exec(exe, tmp_path)
+    np.ascontiguousarray(pa["f"] if obj is None else obj, dtype=np.float32).tofile(tmp_path / "f.bin")
+    stats = _exec(exe, tmp_path, ["trb"] if trb else [])
     proj = np.fromfile(tmp_path / "proj.bin", dtype=np.float32).reshape(len(sel), pa["n"], pa["nz"])
     return torch.from_numpy(proj), stats
 
@@ -158,6 +159,24 @@ def test_forward_body_on_cpu_plain_128(cpu_build, tmp_path):
 
 
 @pytest.mark.parametrize("name,spec,sel,att_mode", CASES, ids=[c[0] for c in CASES])
+def test_transpose_of_back_projection_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel, att_mode):
+    """The forward body's B^T mode (transpose of the reference back
+    projection's gather) against the oracle's autograd VJP of its reference
+    back projection, in float64."""
+    spec = dict(spec)
+    pa = problem_arrays(spec.pop("n"), spec.pop("n_angles"), seed=9, **spec)
+    v = torch.rand((pa["n"], pa["n"], pa["nz"]), generator=torch.Generator().manual_seed(10), dtype=torch.float64)
+    out, stats = _run(cpu_build["fwd_cpu"], tmp_path, pa, sel, att_mode=att_mode, trb=True, obj=v.numpy())
+    geo = oracle_geometry(pa, torch.float64)
+    y = torch.zeros((len(sel), pa["n"], pa["nz"]), dtype=torch.float64, requires_grad=True)
+    (ref,) = torch.autograd.grad(so.backward(geo, y, torch.tensor(sel)), y, v)
+    errs = [rel_l2(out[k], ref[k]) for k in range(len(sel))]
+    print(f"{name}: B^T, H {stats['H']}, max rel-L2 {max(errs):.1e}")
+    assert stats["nan_partial"] == 0
+    assert max(errs) <= 1e-5
+
+
+@pytest.mark.parametrize("name,spec,sel,att_mode", CASES, ids=[c[0] for c in CASES])
 def test_backward_bodies_on_cpu_match_oracle(cpu_build, tmp_path, name, spec, sel, att_mode):
     """Field + gather (reference back projection, mode 0) and field + exact
     adjoint gather (mode 1) of seeded projections, against the oracle's
@@ -204,3 +223,34 @@ def test_backward_bodies_on_cpu_plain_128(cpu_build, tmp_path):
           f"reference rel-L2 {e_ref:.1e}, adjoint rel-L2 {e_adj:.1e}")
     assert bool(torch.isfinite(ref_cpu).all()) and bool(torch.isfinite(adj_cpu).all())
     assert e_ref <= 1e-5 and e_adj <= 1e-5
+
+
+GRAD_CASES = [c for c in CASES if c[3]]
+
+
+@pytest.mark.parametrize("name,spec,sel,att_mode", GRAD_CASES, ids=[c[0] for c in GRAD_CASES])
+def test_projector_vjp_on_cpu_matches_oracle_autograd(cpu_build, tmp_path, name, spec, sel, att_mode):
+    """The differentiable projector's VJP (field with the mu-gradient parts,
+    prefix of chunk totals, gather mode 2) against torch autograd through the
+    oracle's forward, in float64: gradients w.r.t. the activity and mu."""
+    spec = dict(spec)
+    pa = problem_arrays(spec.pop("n"), spec.pop("n_angles"), seed=13, **spec)
+    _write_problem(tmp_path, pa, sel, att_mode=att_mode)
+    gen = torch.Generator().manual_seed(14)
+    u = torch.rand((len(sel), pa["n"], pa["nz"]), generator=gen, dtype=torch.float64)
+    f = torch.from_numpy(pa["f"]).double() + 0.1 * torch.rand(pa["f"].shape, generator=gen, dtype=torch.float64)
+    u.numpy().astype(np.float32).tofile(tmp_path / "p.bin")
+    f.numpy().astype(np.float32).tofile(tmp_path / "f.bin")
+    _exec(cpu_build["bwd_cpu"], tmp_path, ["grad"])
+    shape = (pa["n"], pa["n"], pa["nz"])
+    gf = torch.from_numpy(np.fromfile(tmp_path / "grad_f.bin", dtype=np.float32).reshape(shape))
+    gm = torch.from_numpy(np.fromfile(tmp_path / "grad_mu.bin", dtype=np.float32).reshape(shape))
+    geo = oracle_geometry(pa, torch.float64)
+    fr = f.clone().requires_grad_(True)
+    mr = torch.from_numpy(pa["mu"]).double().requires_grad_(True)
+    y = so.forward_autograd(geo, fr, mr, torch.tensor(sel))
+    rf, rm = torch.autograd.grad(y, (fr, mr), u)
+    ef, em = rel_l2(gf, rf), rel_l2(gm, rm)
+    print(f"{name}: VJP rel-L2 d/df {ef:.1e}, d/dmu {em:.1e}")
+    assert bool(torch.isfinite(gf).all()) and bool(torch.isfinite(gm).all())
+    assert ef <= 1e-5 and em <= 1e-5
