@@ -109,11 +109,12 @@ __device__ __forceinline__ T ld_uniform_struct(const T *p) {
   return u.t;
 }
 
-// Chunk of grid slice z, centre-out (mid, mid+1, mid-1, ...): the chunks
-// through the middle of the padded frame, which hold the most planes inside
-// the object, are dispatched first and the light edge chunks last, so the
-// tail of a launch is made of short blocks.  A bijection of [0, C).
-__device__ __forceinline__ int chunk_centre_out(int z, int C) {
+// Index of grid slice z taken centre-out (mid, mid+1, mid-1, ...): the
+// chunks through the middle of the padded frame, which hold the most planes
+// inside the object, and the central column tiles are dispatched first and
+// the light edge ones last, so the tail of a launch is made of short blocks.
+// A bijection of [0, C).
+__device__ __forceinline__ int centre_out(int z, int C) {
   const int mid = (C - 1) / 2;
   return (z & 1) ? mid + (z + 1) / 2 : mid - z / 2;
 }

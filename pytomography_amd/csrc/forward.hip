@@ -52,12 +52,13 @@ __global__ __launch_bounds__(1024) void att_field_kernel(Geo<R> g, Tables<R> tb,
 template <typename R, int RZ, int QS, int H, bool TRB, bool AF>
 __global__ __launch_bounds__(512) void fwd_kernel(FwdJob<R> J) {
   extern __shared__ __align__(16) unsigned char smem[];
-  fwd_body<R, RZ, QS, H, TRB, AF>(J, blockIdx.x, blockIdx.y, chunk_centre_out(blockIdx.z, J.g.C), smem);
+  // tiles and chunks are taken centre-out: the heaviest blocks are dispatched first
+  fwd_body<R, RZ, QS, H, TRB, AF>(J, blockIdx.x, centre_out(blockIdx.y, J.g.ntiles),
+                                  centre_out(blockIdx.z, J.g.C), smem);
 }
 
 // Batched: block b belongs to job find_job(b); within a job the blocks run
-// over (angle, tile) fastest and then over the chunks centre-out, as in the
-// single-problem grid.
+// over (angle, tile) fastest and then over the chunks centre-out.
 template <typename R, int RZ, int QS, int H, bool AF>
 __global__ __launch_bounds__(512) void fwd_batch_kernel(const FwdJob<R> *__restrict__ jobs,
                                                         const int *__restrict__ starts, int njobs) {
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(512) void fwd_batch_kernel(const FwdJob<R> *__restr
   const FwdJob<R> J = ld_uniform_struct(jobs + j);
   const int local = b - ld_uniform(starts + j);
   const int per_chunk = J.n * J.g.ntiles;
-  const int c = chunk_centre_out(local / per_chunk, J.g.C), r = local % per_chunk;
+  const int c = centre_out(local / per_chunk, J.g.C), r = local % per_chunk;
   fwd_body<R, RZ, QS, H, false, AF>(J, r % J.n, r / J.n, c, smem);
 }
 
