@@ -149,3 +149,14 @@ def test_synthetic_inputs_are_seeded():
     assert np.array_equal(a, b) and a.max() > 0.1 and a.min() == 0
     mu = synthetic.attenuation_map(32, 3)
     assert set(np.unique(mu)) <= {0.0, np.float32(0.15), np.float32(0.25)}
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says so."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
