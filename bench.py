@@ -54,13 +54,13 @@ def build_problem(n: int, n_angles: int, att: bool, psf: bool, seed: int, device
         tr.append(SPECTPSFTransform(SPECTPSFMeta(synthetic.PSF_PARAMS)))
     sm = SPECTSystemMatrix(tr, [], om, pm)
     expected = sm.forward(f_true.to(device)) * 1e3
-    gen = torch.Generator(device=device).manual_seed(seed)
+    # counts drawn on the host from the checked rates (synthetic.poisson_data)
     if att:
         scatter = torch.full_like(expected, 0.05 * float(expected.mean()))
-        data = torch.poisson(expected + scatter, generator=gen)
+        data = synthetic.poisson_data(expected + scatter, seed)
     else:
         scatter = None
-        data = torch.poisson(expected, generator=gen)
+        data = synthetic.poisson_data(expected, seed)
     return dict(sm=sm, f_true=f_true, mu=mu, angles=angles, data=data, scatter=scatter)
 
 
@@ -77,38 +77,60 @@ def algorithmic_bytes(n: int, a_s: int, att: bool):
     return fwd, bwd, upd, ratio
 
 
-def cpu_baseline(n: int, n_angles: int, att: bool, psf: bool, seed: int, budget_s: float):
-    """Oracle (op-for-op torch-CPU restatement of the reference path) timed on
-    whole OSEM sub-iterations until ``budget_s`` is spent; returns it/s."""
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo, as lscpu reports it)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(prob: dict, n_subsets: int, att: bool, psf: bool, reps: int = 3):
+    """The oracle (op-for-op torch-CPU restatement of the reference path,
+    including its per-angle loop and per-call mu rotation) run on the same
+    workload as the GPU: the same counts, additive term and attenuation map,
+    each subset with its own sensitivity image (computed first, untimed).
+    ``reps`` full OSEM iterations (all subsets) are timed one by one from the
+    uniform image; the baseline is 1 / median iteration time."""
     from oracle import spect_oracle as so
     from pytomography_amd import synthetic
 
-    torch.set_num_threads(max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "64")))))
-    mu = torch.from_numpy(synthetic.attenuation_map(n, seed)) if att else None
-    g = so.make_geometry((n, n, n), (synthetic.DX,) * 3, synthetic.angles(n_angles),
-                         radii=np.full(n_angles, synthetic.RADIUS), mu=mu,
+    threads = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "64"))))
+    torch.set_num_threads(threads)
+    n = int(prob["f_true"].shape[0])
+    n_angles = len(prob["angles"])
+    g = so.make_geometry((n, n, n), (synthetic.DX,) * 3, prob["angles"],
+                         radii=np.full(n_angles, synthetic.RADIUS),
+                         mu=prob["mu"] if att else None,
                          psf=so.PSFModel(synthetic.PSF_PARAMS) if psf else None)
-    subsets = g.subsets(8)
+    data = prob["data"].detach().cpu().float()
+    additive = (prob["scatter"].detach().cpu().float() if prob["scatter"] is not None
+                else torch.zeros_like(data))
+    subsets = g.subsets(n_subsets)
+    t0 = time.perf_counter()
+    norms = [so.sensitivity(g, sidx) for sidx in subsets]
+    sens_s = time.perf_counter() - t0
     f = torch.ones((n, n, n))
-    gen = torch.Generator().manual_seed(seed)
-    data = torch.poisson(torch.full((n_angles, n, n), 50.0), generator=gen)
-    norm = so.sensitivity(g, subsets[0])
-    done, t0 = 0, time.perf_counter()
-    while True:
-        sidx = subsets[done % 8]
-        pp = so.forward(g, f, sidx)
-        b = so.backward(g, data[sidx] / (pp + so.DELTA), sidx)
-        f = f + f * (1 / (norm + so.DELTA)) * (b - norm)
-        done += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or done >= 8:
-            break
-    per_sub = el / done
-    return dict(value=1.0 / (8 * per_sub), unit="OSEM it/s", cores=torch.get_num_threads(),
-                kind="port",
-                sample=f"{done} OSEM sub-iteration(s) of {len(subsets[0])} angles (forward + reference "
-                       f"back projection + update) of the same {n}^3 x {n_angles} workload, "
-                       f"{el:.1f} s, scaled x8 to one iteration")
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for k, sidx in enumerate(subsets):
+            pp = so.forward(g, f, sidx) + additive[sidx]
+            b = so.backward(g, data[sidx] / (pp + so.DELTA), sidx)
+            f = f + f * (1 / (norms[k] + so.DELTA)) * (b - norms[k])
+            f[f <= 0] = 0
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return dict(value=1.0 / med, unit="OSEM it/s", cores=threads, kind="port",
+                cpu_model=cpu_model(), iteration_s=times, sensitivity_s=sens_s,
+                sample=f"median of {reps} full OSEM iterations ({n_subsets} subsets of "
+                       f"{len(subsets[0])} angles: forward + additive term + reference back "
+                       f"projection + update) of the same {n}^3 x {n_angles} workload and data, "
+                       f"after the {n_subsets} sensitivity images ({sens_s:.1f} s, untimed)")
 
 
 def bench_batch(args, device):
@@ -154,7 +176,7 @@ def bench_batch(args, device):
             f_true = torch.from_numpy(synthetic.ellipsoid_phantom(n, seed)).to(device)
             exp = sm.forward(f_true) * 1e3
             s = torch.full_like(exp, 0.05 * float(exp.mean()))
-            g = torch.poisson(exp + s, generator=torch.Generator(device=device).manual_seed(seed))
+            g = synthetic.poisson_data(exp + s, seed)
             out.append(PoissonLogLikelihood(sm, g, additive_term=s))
         return out
 
@@ -410,8 +432,9 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="object size (128; 192 for unrolled)")
     ap.add_argument("--angles", type=int, default=None, help="angles (120; 180 for unrolled)")
     ap.add_argument("--subsets", type=int, default=None, help="subsets (8; 4 for unrolled)")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain-leg", action="store_true",
+                    help="skip the no-attenuation, no-PSF leg of the default workload")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
@@ -440,19 +463,6 @@ def main():
     if args.workload == "aux":
         bench_aux(args, device)
         return
-    att = psf = args.workload == "att_psf"
-    n, na, ns = args.n, args.angles, args.subsets
-    if not att and n * n * n * na > 64 * 64 * 64 * 60:
-        # A run of this workload at 128^3 x 120 ended in torch.poisson's
-        # non-negative-rate device assert while simulating the data; the
-        # cause is not identified, so it stays off the GPU at that size.
-        print("bench.py: --workload plain above 64^3 x 60 is disabled (an abort while "
-              "simulating the data at 128^3 x 120 has no identified cause)", file=sys.stderr)
-        sys.exit(2)
-
-    from pytomography_amd.algorithms import OSEM
-    from pytomography_amd.likelihoods import PoissonLogLikelihood
-
     if args.copy_mib > 0:
         src = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=device).fill_(1.0)
         dst = torch.empty_like(src)
@@ -460,6 +470,70 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    att = psf = args.workload == "att_psf"
+    n, na, ns = args.n, args.angles, args.subsets
+    head = measure_osem(n, na, ns, att, psf, device, args.steps, args.warmup,
+                        cpu=not args.no_cpu_baseline, tag=args.workload)
+    plain = None
+    if att and not args.no_plain_leg:
+        # the same geometry without attenuation and PSF (the reference's
+        # CPU-runnable case), timed the same way with its own CPU baseline
+        plain = measure_osem(n, na, ns, False, False, device, args.steps, args.warmup,
+                             cpu=not args.no_cpu_baseline, tag="plain")
+    a_s = na // ns
+    out = {
+        "metric": METRIC,
+        "value": head["its"],
+        "unit": "OSEM it/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / head["its"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantom, Poisson counts drawn on the host)",
+        "config": {
+            "workload": (f"SPECT OSEM {n}^3 x {na} angles, "
+                         + ("attenuation + PSF" if att else "no attenuation, no PSF")
+                         + f", {ns} subsets"),
+            "n": n, "angles": na, "subsets": ns, "attenuation": att, "psf": psf,
+            **head["config"],
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": head["kernel"], "achieved": head["achieved"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": head["achieved"] / HBM_PEAK_GBS,
+            "traffic": head["traffic"], "algorithmic_bytes_per_launch": head["bytes_launch"],
+            "launch_ms": head["launch_ms"],
+            "algorithmic_bytes_unit": f"15 angles x (8 N^3 + 4 N^2) bytes, N = {n}" if att else
+                                      f"{a_s} angles x (4 N^3 + 4 N^2) bytes, N = {n}",
+        },
+        "cpu_baseline": head["cpu"],
+    }
+    if plain is not None:
+        out["config"]["plain_leg"] = {
+            "workload": f"SPECT OSEM {n}^3 x {na} angles, no attenuation, no PSF, {ns} subsets",
+            "value": plain["its"], "unit": "OSEM it/s",
+            "roofline_frac": plain["achieved"] / HBM_PEAK_GBS, "kernel": plain["kernel"],
+            "launch_ms": plain["launch_ms"], **plain["config"],
+            "cpu_baseline": plain["cpu"],
+        }
+    print(json.dumps(out))
+
+
+def measure_osem(n: int, na: int, ns: int, att: bool, psf: bool, device, steps: int, warmup: int,
+                 cpu: bool, tag: str) -> dict:
+    """One OSEM workload: throughput over ``steps`` iterations after ``warmup``,
+    the per-launch device time of the forward projection and of the fused
+    back projection + update (HIP events on the launch stream), the float32
+    iterate against the float64 kernels, and the CPU baseline."""
+    from pytomography_amd.algorithms import OSEM
+    from pytomography_amd.likelihoods import PoissonLogLikelihood
+    import pytomography_amd
+
+    print(f"[bench] {tag}: {n}^3 x {na}, att {att}, psf {psf}: building the problem",
+          file=sys.stderr, flush=True)
     prob = build_problem(n, na, att, psf, seed=0, device=device)
     sm = prob["sm"]
     lk = PoissonLogLikelihood(sm, prob["data"], additive_term=prob["scatter"])
@@ -475,17 +549,18 @@ def main():
         for k in range(ns):
             alg._fused_subiteration(0, k)
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         osem_iteration()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         osem_iteration()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    its = args.steps / elapsed
+    its = steps / elapsed
     f32_iterate = alg.object_prediction.detach().clone()
-    n_done = args.warmup + args.steps
+    n_done = warmup + steps
+    print(f"[bench] {tag}: {its:.1f} OSEM it/s", file=sys.stderr, flush=True)
 
     # Per-entry-point timing with HIP events on the launch stream: each entry
     # point is launched for every subset, REPS times back to back between two
@@ -524,13 +599,12 @@ def main():
     fwd_b, bwd_b, upd_b, ratio_b = algorithmic_bytes(n, a_s, att)
     # Dominant kernel: the forward projection (fwd_kernel + its chunk reduce),
     # timed above with HIP events on the stream it is launched on.
-    kern, bytes_launch, ms = "fwd_kernel + fwd_reduce_kernel (spect_forward, 15 angles)", fwd_b, fwd_ms
-    achieved = bytes_launch / (ms * 1e-3) / 1e9
+    kern = f"fwd_kernel + fwd_reduce_kernel (spect_forward, {a_s} angles)"
+    achieved = fwd_b / (fwd_ms * 1e-3) / 1e9
     iter_bytes = ns * (fwd_b + bwd_b + upd_b + ratio_b)
 
     # The same number of iterations with the float64 kernels (untimed): the
     # float32 iterate's error against an FP64 run of the same path.
-    import pytomography_amd
     pytomography_amd.set_dtype(torch.float64)
     try:
         lk64 = PoissonLogLikelihood(sm, prob["data"].double(), additive_term=prob["scatter"])
@@ -550,32 +624,21 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            entry = json.load(fh).get(f"{args.workload}_{n}_{na}_{ns}")
+            entry = json.load(fh).get(f"{tag}_{n}_{na}_{ns}")
         if entry:
             traffic = entry.get("spect_forward")
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(n, na, att, psf, seed=0, budget_s=args.cpu_budget)
-
-    out = {
-        "metric": METRIC,
-        "value": its,
-        "unit": "OSEM it/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded ellipsoid phantom, Poisson counts)",
-        "config": {
-            "workload": (f"SPECT OSEM {n}^3 x {na} angles, "
-                         + ("attenuation + PSF" if att else "no attenuation, no PSF")
-                         + f", {ns} subsets"),
-            "n": n, "angles": na, "subsets": ns, "attenuation": att, "psf": psf,
+    cpu_res = None
+    if cpu:
+        print(f"[bench] {tag}: CPU baseline (oracle on the host)", file=sys.stderr, flush=True)
+        cpu_res = cpu_baseline(prob, ns, att, psf)
+        print(f"[bench] {tag}: CPU {cpu_res['value']:.4f} OSEM it/s", file=sys.stderr, flush=True)
+    del alg, lk, prob, sm
+    torch.cuda.empty_cache()
+    return dict(
+        its=its, kernel=kern, achieved=achieved, bytes_launch=fwd_b, launch_ms=fwd_ms,
+        traffic=traffic, cpu=cpu_res,
+        config={
             "projections_per_s": its * na,
             "iteration_algorithmic_bytes": iter_bytes,
             "iteration_hbm_frac": iter_bytes * its / (HBM_PEAK_GBS * 1e9),
@@ -584,15 +647,7 @@ def main():
             "iterations_compared": n_done,
             "forward_ms_per_subset": fwd_ms,
             "backward_update_ms_per_subset": bwd_ms,
-        },
-        "roofline": {
-            "bound": "hbm", "kernel": kern, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "algorithmic_bytes_per_launch": bytes_launch, "launch_ms": ms,
-        },
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out))
+        })
 
 
 if __name__ == "__main__":

@@ -212,6 +212,9 @@ typedef struct spect_batch_problem {
   size_t workspace_bytes;
 } spect_batch_problem;
 
+/* Problems may share read-only inputs (plan, mu, attenuation data, g, s,
+ * norm); each needs its own f and workspace: spect_batch_create returns
+ * SPECT_E_ARG when two of them overlap. */
 int spect_batch_create(const spect_batch_problem *probs, int n, int dtype, double delta,
                        double floor_value, spect_batch **out);
 /* n_iters full iterations (all subsets, all problems).  use_graph != 0 captures

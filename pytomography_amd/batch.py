@@ -63,7 +63,12 @@ class BatchedOSEM:
             pred = [torch.empty_like(gk) for gk in g]
             mu = sm._mu(device, dt)
             tab = sm._att_table(device, dt)
-            ws, nbytes = sm._workspace(device, dt, max(int(t.numel()) for t in idx))
+            # each problem gets a workspace of its own: problems may share a
+            # system matrix (one scanner geometry, several datasets), and the
+            # jobs of one launch run concurrently
+            nbytes = int(lib.spect_workspace_bytes(plan.handle, _lib.dtype_code(dt),
+                                                   max(int(t.numel()) for t in idx)))
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
 
             def ptrs(ts):
                 return (ctypes.c_void_p * n_subsets)(*[None if t is None else t.data_ptr() for t in ts])

@@ -188,8 +188,8 @@ int check_arrays(std::initializer_list<const void *> arrays) {
 // or rows built now in the workspace (indexed by position in angle_idx).
 template <typename R>
 int att_rows(const spect_plan *p, const Geo<R> &g, const R *mu, const void *table, bool field,
-             const int32_t *idx, int n, unsigned char *ws, const Ws &l, hipStream_t st,
-             const double **att, bool *global, const R **attf) {
+             const int32_t *idx, int n, double *rows, hipStream_t st, const double **att,
+             bool *global, const R **attf) {
   *att = nullptr;
   *global = false;
   *attf = nullptr;
@@ -204,9 +204,26 @@ int att_rows(const spect_plan *p, const Geo<R> &g, const R *mu, const void *tabl
     *global = true;
     return SPECT_OK;
   }
-  double *rows = reinterpret_cast<double *>(ws + l.att_off);
   HIP_TRY(launch_att_table<R>(g, tables<R>(p), mu, idx, n, rows, st));
   *att = rows;
+  return SPECT_OK;
+}
+
+// Forward projection with its scratch given explicitly: rows (FP64
+// attenuation rows, used only with mu and no table) and part (the chunk
+// partials).
+template <typename R>
+int forward_core(const spect_plan *p, const R *f, const R *mu, const void *table, bool field,
+                 const int32_t *idx, int n, R *proj, const R *gm, const R *s, R *pred, R *ratio,
+                 R delta, double *rows, R *part, hipStream_t st, bool trb) {
+  const Geo<R> g = geo<R>(p);
+  const double *att;
+  bool global;
+  const R *attf;
+  int rc = att_rows<R>(p, g, mu, table, field, idx, n, rows, st, &att, &global, &attf);
+  if (rc) return rc;
+  HIP_TRY(launch_forward<R>(g, tables<R>(p), f, mu, att, global, attf, idx, n, part, trb, st));
+  HIP_TRY(launch_forward_reduce<R>(g, part, n, proj, gm, s, pred, ratio, delta, st));
   return SPECT_OK;
 }
 
@@ -214,17 +231,10 @@ template <typename R>
 int forward_impl(const spect_plan *p, const R *f, const R *mu, const void *table, bool field,
                  const int32_t *idx, int n, R *proj, const R *gm, const R *s, R *pred, R *ratio,
                  R delta, unsigned char *ws, hipStream_t st, bool trb = false) {
-  const Geo<R> g = geo<R>(p);
   const Ws l = ws_layout(p, sizeof(R) == 8 ? SPECT_F64 : SPECT_F32, n);
-  const double *att;
-  bool global;
-  const R *attf;
-  int rc = att_rows<R>(p, g, mu, table, field, idx, n, ws, l, st, &att, &global, &attf);
-  if (rc) return rc;
-  R *part = reinterpret_cast<R *>(ws + l.part_off);
-  HIP_TRY(launch_forward<R>(g, tables<R>(p), f, mu, att, global, attf, idx, n, part, trb, st));
-  HIP_TRY(launch_forward_reduce<R>(g, part, n, proj, gm, s, pred, ratio, delta, st));
-  return SPECT_OK;
+  return forward_core<R>(p, f, mu, table, field, idx, n, proj, gm, s, pred, ratio, delta,
+                         reinterpret_cast<double *>(ws + l.att_off),
+                         reinterpret_cast<R *>(ws + l.part_off), st, trb);
 }
 
 template <typename R>
@@ -237,7 +247,8 @@ int backward_impl(const spect_plan *p, int mode, int out_mode, const R *proj, co
   const double *att;
   bool global;
   const R *attf;
-  int rc = att_rows<R>(p, g, mu, table, field, idx, n, ws, l, st, &att, &global, &attf);
+  int rc = att_rows<R>(p, g, mu, table, field, idx, n, reinterpret_cast<double *>(ws + l.att_off),
+                       st, &att, &global, &attf);
   if (rc) return rc;
   R *W = reinterpret_cast<R *>(ws + l.w_off);
   // With the attenuation field the field kernel reads per-plane factors and
@@ -461,6 +472,29 @@ int spect_batch_create(const spect_batch_problem *probs, int n, int dtype, doubl
     if (pb.workspace_bytes < ws_layout(pb.plan, dtype, cmax).total)
       return fail(SPECT_E_WORKSPACE, "batch problem " + std::to_string(i) + ": workspace too small");
   }
+  // The jobs of one launch run concurrently: a workspace (partials, ratio,
+  // field) or an iterate shared by two problems would be written by both.
+  {
+    const size_t rs = dtype == SPECT_F64 ? 8 : 4;
+    struct Span { uintptr_t lo, hi; int prob; const char *what; };
+    std::vector<Span> spans;
+    for (int i = 0; i < n; ++i) {
+      const spect_batch_problem &pb = probs[i];
+      int cmax = 0;
+      for (int k = 0; k < S; ++k) cmax = std::max(cmax, (int)pb.counts[k]);
+      const uintptr_t w = (uintptr_t)pb.workspace, f = (uintptr_t)pb.f;
+      spans.push_back({w, w + ws_layout(pb.plan, dtype, cmax).total, i, "workspace"});
+      spans.push_back({f, f + (size_t)pb.plan->N * pb.plan->N * pb.plan->Nz * rs, i, "iterate f"});
+    }
+    for (size_t a = 0; a < spans.size(); ++a)
+      for (size_t c = a + 1; c < spans.size(); ++c)
+        if (spans[a].lo < spans[c].hi && spans[c].lo < spans[a].hi)
+          return fail(SPECT_E_ARG, std::string("batch problems ") + std::to_string(spans[a].prob) +
+                                       " (" + spans[a].what + ") and " +
+                                       std::to_string(spans[c].prob) + " (" + spans[c].what +
+                                       ") overlap in device memory; each problem needs its own "
+                                       "workspace and iterate");
+  }
   auto *b = new spect_batch();
   b->dtype = dtype;
   b->n_subsets = S;
@@ -540,6 +574,8 @@ int spect_plan_create(const spect_geom *g, spect_plan **out) {
   if (g->psf_kr > 0 && (g->n_radii <= 0 || !g->radius_index || !g->taps_r ||
                         (g->psf_kz > 0 && !g->taps_z)))
     return fail(SPECT_E_ARG, "PSF tables missing");
+  if (g->psf_kr > 0 && g->psf_kz > 0 && g->taps_r64 && !g->taps_z64)
+    return fail(SPECT_E_ARG, "float64 PSF tables: taps_z64 is required with taps_r64 when psf_kz > 0");
   const int hmax = std::max(g->psf_kr, g->psf_kz) / 2;
   int H = g->psf_kr > 0 ? (hmax + 1) / 2 * 2 : 0;  // even half widths are compiled
   if (g->psf_kr > 0 && H == 0) H = 2;

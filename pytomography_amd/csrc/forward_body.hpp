@@ -126,7 +126,11 @@ template <int RZ, int H> __device__ __forceinline__ int lds_stride(int runs) {
 // (spect_system_matrix.py:196): rotated-frame pixel (i, j) collects every
 // object pixel whose theta_b sample has (i, j) among its bilinear taps, with
 // that tap's weight -- the body then computes B^T v (attenuation, PSF, sum).
-constexpr int kTrbMax = 12;  // contributors per rotated-frame pixel (<= 9 in exact arithmetic)
+// Contributors per rotated-frame pixel: the search below visits a 4 x 4 block
+// of object pixels, so a list of 16 can never overflow (<= 9 occur in exact
+// arithmetic: lattice points of a side-2 square).
+constexpr int kTrbSearch = 4;
+constexpr int kTrbMax = kTrbSearch * kTrbSearch;
 
 template <typename R, int RZ, int QS, int H, bool TRB, bool AF>
 __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int c,
@@ -239,10 +243,10 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
       const int j = j0 + tid;
       const int Xb = int(floor(iy - R(1.5))), Yb = int(floor(ix - R(1.5)));
       int cnt = 0;
-      for (int dx_ = 0; dx_ < 4; ++dx_) {
+      for (int dx_ = 0; dx_ < kTrbSearch; ++dx_) {
         const int X = Xb + dx_;
         if ((unsigned)(X - g.pad) >= (unsigned)g.N) continue;
-        for (int dy_ = 0; dy_ < 4; ++dy_) {
+        for (int dy_ = 0; dy_ < kTrbSearch; ++dy_) {
           const int Y = Yb + dy_;
           if ((unsigned)(Y - g.pad) >= (unsigned)g.N) continue;
           R jc, ic;
@@ -257,11 +261,10 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
           if (c0 == j) wc = R(1) - tx;
           else if (c0 + 1 == j) wc = tx;
           else continue;
-          if (cnt < kTrbMax) {
-            trb_off[b][tid][cnt] = (uint32_t)((X - g.pad) * g.N + (Y - g.pad)) * (uint32_t)Nz;
-            trb_w[b][tid][cnt] = wr * wc;
-            ++cnt;
-          }
+          // cnt < kTrbMax: at most one entry per visited (X, Y)
+          trb_off[b][tid][cnt] = (uint32_t)((X - g.pad) * g.N + (Y - g.pad)) * (uint32_t)Nz;
+          trb_w[b][tid][cnt] = wr * wc;
+          ++cnt;
         }
       }
       // a contributor's forward sample lies within sqrt(2) px of the object,

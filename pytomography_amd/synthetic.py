@@ -72,5 +72,17 @@ PSF_PARAMS = (0.03, 0.1)
 
 
 def poisson_data(expected: torch.Tensor, seed: int) -> torch.Tensor:
+    """Poisson counts with mean ``expected``, drawn on the host (seeded CPU
+    generator) and returned on ``expected``'s device and dtype.
+
+    The rates are checked first: a NaN, infinite or negative rate raises
+    ``ValueError`` naming the offending count, instead of reaching a sampler
+    whose device build aborts on it."""
+    host = expected.detach().to("cpu", torch.float64)
+    bad_finite = int((~torch.isfinite(host)).sum())
+    bad_sign = int((host < 0).sum())
+    if bad_finite or bad_sign:
+        raise ValueError(f"Poisson rates must be finite and >= 0: {bad_finite} non-finite and "
+                         f"{bad_sign} negative of {host.numel()}")
     gen = torch.Generator(device="cpu").manual_seed(seed)
-    return torch.poisson(expected.detach().cpu().double(), generator=gen).to(expected.dtype)
+    return torch.poisson(host, generator=gen).to(device=expected.device, dtype=expected.dtype)

@@ -1,10 +1,12 @@
 """GPU parity at full size against the CPU oracle: the 128^3 object with 120
 angles, attenuation (water body, two bone ellipsoids) and the depth-dependent
 PSF (K = 29), one 15-angle subset of 8 -- the launches the OSEM benchmark
-times -- and the 192^3 object with 180 angles (K = 37, the unrolled BSREM
-geometry), one 9-angle subset of 20: forward projection, reference back
-projection, and one fused OSEM sub-iteration (forward, Poisson ratio with the
-additive term, back projection and update) from a non-uniform iterate.
+times; the same 128^3 x 120 geometry with neither attenuation nor PSF (the
+reference's CPU-runnable case, subset 5 of 8); and the 192^3 object with 180
+angles (K = 37, the unrolled BSREM geometry), one 9-angle subset of 20:
+forward projection, reference back projection, and one fused OSEM
+sub-iteration (forward, Poisson ratio with the additive term, back projection
+and update) from a non-uniform iterate.
 
 Bar (FP32): relative L2 <= 1e-5 per projection and for the back projection and
 the updated iterate; each result also reports its error against the float64
@@ -20,16 +22,18 @@ from tests.helpers import oracle_geometry, problem_arrays, product_system_matrix
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
-# (side, angles, subsets, subset tested)
-SIZES = [(128, 120, 8, 3), (192, 180, 20, 7)]
+# (side, angles, subsets, subset tested, attenuation + PSF)
+SIZES = [(128, 120, 8, 3, True), (128, 120, 8, 5, False), (192, 180, 20, 7, True)]
 
 
-@pytest.fixture(scope="module", params=SIZES, ids=lambda p: f"{p[0]}^3x{p[1]}")
+@pytest.fixture(scope="module", params=SIZES,
+                ids=lambda p: f"{p[0]}^3x{p[1]}" + ("_att_psf" if p[4] else "_plain"))
 def problem(request):
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    n, na, n_subsets, subset = request.param
-    pa = problem_arrays(n, na, seed=0, att=True, psf=True)
+    n, na, n_subsets, subset, att_psf = request.param
+    print(f"[fullsize] building {n}^3 x {na} att+psf={att_psf}", flush=True)
+    pa = problem_arrays(n, na, seed=0, att=att_psf, psf=att_psf)
     sm = product_system_matrix(pa)
     sm.set_n_subsets(n_subsets)
     g32 = oracle_geometry(pa, torch.float32)
@@ -42,8 +46,10 @@ def test_headline_subset_forward_and_back_projection(problem):
     pa, sm, g32, g64, sidx, _, subset = problem
     dev = torch.device("cuda")
     f = torch.from_numpy(pa["f"])
-    assert sm._att_table(dev, torch.float32).att_field  # the benchmark's attenuation mode
-    tag = f"{pa['n']}^3 x {len(pa['angles'])} att+PSF (H {sm._plan(dev).psf_half}) subset {subset}"
+    if pa["mu"] is not None:
+        assert sm._att_table(dev, torch.float32).att_field  # the benchmark's attenuation mode
+    kind = "att+PSF" if pa["psf"] else "plain"
+    tag = f"{pa['n']}^3 x {len(pa['angles'])} {kind} (H {sm._plan(dev).psf_half}) subset {subset}"
     out = sm.forward(f.to(dev), subset).cpu()
     ref = so.forward(g32, f, sidx)
     ref64 = so.forward(g64, f.double(), sidx)

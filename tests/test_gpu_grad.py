@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 import torch
 
+from pytomography_amd import synthetic
 from oracle import spect_oracle as so
 from tests.helpers import oracle_geometry, problem_arrays, product_system_matrix, rel_l2
 
@@ -29,8 +30,7 @@ def _data(sm, f_true, mu, scale=100.0):
     with torch.no_grad():
         hf = sm.forward_with_mu(f_true, mu) * scale
     s = torch.full_like(hf, 0.05 * float(hf.mean()))
-    gen = torch.Generator(device=hf.device).manual_seed(3)
-    g = torch.poisson(hf + s, generator=gen)
+    g = synthetic.poisson_data(hf + s, 3)
     return g, s
 
 

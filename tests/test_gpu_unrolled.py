@@ -13,6 +13,7 @@ from __future__ import annotations
 import pytest
 import torch
 
+from pytomography_amd import synthetic
 from oracle import spect_oracle as so
 from tests.helpers import oracle_geometry, problem_arrays, product_system_matrix, rel_l2
 
@@ -140,7 +141,7 @@ def test_unrolled_bsrem_fp32_gradient_vs_fp64(cuda):
     with torch.no_grad():
         expected = sm.forward(f_true) * 50
     s = torch.full_like(expected, 0.05 * float(expected.mean()))
-    data = torch.poisson(expected + s, generator=torch.Generator(device=cuda).manual_seed(40))
+    data = synthetic.poisson_data(expected + s, 40)
     grads = {}
     for dt in (torch.float64, torch.float32):
         un = UnrolledBSREM(sm, RelativeDifferencePrior(beta=0.1, gamma=2.0), additive_term=s.to(dt),
