@@ -273,6 +273,112 @@ def _bsrem_problem():
     return g, f_true, data, s
 
 
+def _dense_plain(angles, shape=(8, 8, 4)):
+    """Dense forward H and reference back projection B (numpy brute force)."""
+    H = brute.dense_matrix(lambda e: brute.forward(e, angles, 0.48), shape)
+    B = brute.dense_matrix(lambda p: brute.backward_ref(p, angles, 0.48, shape),
+                           (len(angles), shape[1], shape[2]))
+    return H, B
+
+
+def _em_counts(H, shape, seed):
+    rng = np.random.default_rng(seed)
+    f_true = rng.random(shape) + 0.2
+    expected = H @ f_true.ravel() * 50
+    s = np.full(expected.shape, 0.05 * expected.mean())
+    data = rng.poisson(expected + s).astype(np.float64)
+    x0 = np.ones(int(np.prod(shape)))
+    x0[::7] = 1e-6   # below BSREM's floor: the first update must lift them to 1e-4
+    return data, s, x0
+
+
+def test_bsrem_one_subset_without_prior_is_floored_mlem():
+    """BSREM (preconditioned_gradient_ascent.py:436-474) with one subset and
+    beta = 0: w = 1 and sum_m n_m = n, so D = 1/(n + delta) and each step is
+    the MLEM step (written here on the dense brute-force matrices), followed
+    by the 1e-4 floor (:452, :122)."""
+    shape = (8, 8, 4)
+    angles = [0.0, 33.0, 90.0, 141.0, 222.0, 300.0]
+    H, B = _dense_plain(angles, shape)
+    data, s, x0 = _em_counts(H, shape, seed=31)
+    x = x0.copy()
+    sens = B @ np.ones(data.size)
+    floored = 0
+    for _ in range(4):
+        x = x + x / (sens + 1e-11) * (B @ (data / (H @ x + s + 1e-11)) - sens)
+        floored += int((x <= 1e-4).sum())
+        x[x <= 1e-4] = 1e-4
+    g = so.make_geometry(shape, (0.48,) * 3, angles, radii=[6.0] * 6, dtype=torch.float64)
+    out = so.bsrem(g, torch.from_numpy(data.reshape(6, 8, 4)), 4, 1, 0.0, 2.0,
+                   additive=torch.from_numpy(s.reshape(6, 8, 4)),
+                   f0=torch.from_numpy(x0.reshape(shape))).numpy().ravel()
+    assert floored > 0   # the floor acted
+    assert np.abs(out - x).max() < 1e-10 * np.abs(x).max()
+
+
+def test_bsrem_subset_weight_scales_the_sensitivity_sum():
+    """Two subsets {0, 180} and {90, 270} degrees without attenuation or PSF
+    have equal sensitivities n_1 = n_2 (quarter turns), so BSREM's
+    w_k * (n_1 + n_2) with w_k = |subset| / A = 1/2 is n_k and BSREM without
+    a prior steps exactly as OSEM (floored at 1e-4): per subset
+    x <- x + x/(n_k + delta) * (B_k(g_k / (H_k x + s_k)) - n_k), on the dense
+    brute-force matrices.  A missing or inverted w_k changes every step."""
+    shape = (8, 8, 4)
+    angles = [0.0, 90.0, 180.0, 270.0]
+    H, B = _dense_plain(angles, shape)
+    data, s, x0 = _em_counts(H, shape, seed=32)
+    rows = np.arange(data.size).reshape(4, -1)
+    subsets = [rows[[0, 2]].ravel(), rows[[1, 3]].ravel()]
+    sens = [B[:, k] @ np.ones(len(k)) for k in subsets]
+    assert np.abs(sens[0] - sens[1]).max() < 1e-12
+    x = x0.copy()
+    for _ in range(3):
+        for k, (sk, nk) in enumerate(zip(subsets, sens)):
+            x = x + x / (nk + 1e-11) * (B[:, sk] @ (data[sk] / (H[sk] @ x + s[sk] + 1e-11)) - nk)
+            x[x <= 1e-4] = 1e-4
+    g = so.make_geometry(shape, (0.48,) * 3, angles, radii=[6.0] * 4, dtype=torch.float64)
+    out = so.bsrem(g, torch.from_numpy(data.reshape(4, 8, 4)), 3, 2, 0.0, 2.0,
+                   additive=torch.from_numpy(s.reshape(4, 8, 4)),
+                   f0=torch.from_numpy(x0.reshape(shape))).numpy().ravel()
+    assert np.abs(out - x).max() < 1e-10 * np.abs(x).max()
+
+
+def test_bsrem_converges_to_the_penalised_likelihood_kkt_point():
+    """With a diminishing relaxation BSREM converges to the maximiser of
+    L(f) - V(f) over f >= floor (Ahn & Fessler 2003): with three subsets of
+    the six angles and the RDP (beta 0.5), the limit satisfies
+    f * (sum_k [B_k(g_k/(H_k f + s_k)) - B_k 1] - grad V(f)) = 0, i.e. the
+    subset gradients add up to the full one and the prior, scaled by w_k per
+    subset (preconditioned_gradient_ascent.py:115), adds up to beta * phi1
+    once.  B and H are the dense brute-force matrices; grad V is the RDP
+    gradient at beta_scale 1 (pinned separately by finite differences).  The
+    residual shrinks like the relaxation; a prior counted once per subset
+    instead leaves a residual of the size of the prior term itself."""
+    shape = (8, 8, 4)
+    angles = [0.0, 33.0, 90.0, 141.0, 222.0, 300.0]
+    H, B = _dense_plain(angles, shape)
+    rng = np.random.default_rng(33)
+    f_true = rng.random(shape) + 0.2
+    expected = H @ f_true.ravel() * 50
+    s = np.full(expected.shape, 0.05 * expected.mean())
+    data = rng.poisson(expected + s).astype(np.float64)
+    beta, gamma = 0.5, 2.0
+    g = so.make_geometry(shape, (0.48,) * 3, angles, radii=[6.0] * 6, dtype=torch.float64)
+    f = so.bsrem(g, torch.from_numpy(data.reshape(6, 8, 4)), 150, 3, beta, gamma,
+                 additive=torch.from_numpy(s.reshape(6, 8, 4)),
+                 relaxation=lambda j: 1.0 / (1 + j / 4))
+    x = f.numpy().ravel()
+    assert x.min() > 1e-3     # interior point: the floor is not active
+    grad_l = B @ (data / (H @ x + s + 1e-11)) - B @ np.ones(data.size)
+    grad_v = so.rdp_gradient(f.clone(), beta, gamma, g.dr, 1.0).numpy().ravel()
+    scale = np.linalg.norm(x * grad_l)
+    kkt = np.linalg.norm(x * (grad_l - grad_v)) / scale
+    wrong = np.linalg.norm(x * (grad_l - 3 * grad_v)) / scale
+    assert np.linalg.norm(x * grad_v) > 0.5 * scale   # the prior term is not negligible
+    assert kkt < 0.04, kkt
+    assert wrong > 1.0, wrong
+
+
 def test_unrolled_bsrem_values_equal_inplace_bsrem():
     g, f_true, data, s = _bsrem_problem()
     a = so.bsrem(g, data, 2, 3, 0.1, 2.0, additive=s)
