@@ -135,6 +135,27 @@ int spect_forward(const spect_plan *p, int dtype, const void *f, const void *mu,
                   const void *att_table, const int32_t *angle_idx, int n, void *proj,
                   void *workspace, size_t workspace_bytes, void *stream);
 
+/* Largest scratch, in bytes, that one spect_forward_noalloc call may use. */
+#define SPECT_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+
+/* spect_forward without a workspace and without any device allocation: the
+ * chunk partials (and, when mu comes without att_table, the attenuation rows)
+ * live in a scratch array of SPECT_NOALLOC_SCRATCH_BYTES that the library
+ * holds on every device it runs on until the process ends.  Same arguments
+ * and result as spect_forward minus the workspace; nothing in the scratch has
+ * to start at zero.  A call whose scratch would exceed the limit returns
+ * SPECT_E_WORKSPACE before launching anything (project fewer angles per
+ * call: at 128^3 with the 29-tap PSF, 15 angles need 56 MB).  All calls on
+ * one device share that device's scratch, so calls that overlap in time on
+ * one device (different streams or threads) race; calls on one stream do not.
+ * The current device must be the plan's device.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory -- code under hipGraph capture, tight serving loops and
+ * harnesses that forbid hipMalloc. */
+int spect_forward_noalloc(const spect_plan *p, int dtype, const void *f, const void *mu,
+                          const void *att_table, const int32_t *angle_idx, int n, void *proj,
+                          void *stream);
+
 /* Back projection.
  * mode SPECT_BP_REFERENCE replaces SPECTSystemMatrix.backward
  * (spect_system_matrix.py:159-201: PSF (psf.py:216-231), attenuation

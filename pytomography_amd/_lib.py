@@ -20,12 +20,14 @@ ATT_FIELD = 0x100  # OR-ed into dtype: att_table is an attenuation field
 BP_REFERENCE, BP_ADJOINT = 0, 1
 PAD_CIRCULAR, PAD_REPLICATE, PAD_ZEROS = 0, 1, 2
 OUT_STORE, OUT_ADD, OUT_OSEM = 0, 1, 2
+NOALLOC_SCRATCH_BYTES = 128 << 20  # SPECT_NOALLOC_SCRATCH_BYTES
 
 EXPORTED = (
     "spect_last_error", "spect_version", "spect_plan_create", "spect_plan_destroy",
     "spect_plan_dims", "spect_workspace_bytes", "spect_att_table_bytes", "spect_att_table_build",
     "spect_att_field_bytes", "spect_att_field_build",
-    "spect_forward", "spect_backward", "spect_backward_transpose", "spect_osem_subiteration",
+    "spect_forward", "spect_forward_noalloc", "spect_backward", "spect_backward_transpose",
+    "spect_osem_subiteration",
     "spect_grad_workspace_bytes", "spect_forward_vjp",
     "spect_batch_create", "spect_batch_run", "spect_batch_destroy",
     "spect_poisson_ratio", "spect_rdp_gradient", "spect_rdp_hessian_vjp", "spect_rdp_hessian_jvp",
@@ -95,6 +97,8 @@ def _declare(lib):
     lib.spect_att_field_build.argtypes = [vp, ci, vp, vp, vp]
     lib.spect_forward.restype = ci
     lib.spect_forward.argtypes = [vp, ci, vp, vp, vp, vp, ci, vp, vp, sz, vp]
+    lib.spect_forward_noalloc.restype = ci
+    lib.spect_forward_noalloc.argtypes = [vp, ci, vp, vp, vp, vp, ci, vp, vp]
     lib.spect_backward.restype = ci
     lib.spect_backward.argtypes = [vp, ci, ci, ci, vp, vp, vp, vp, ci, vp, vp, vp, dbl, dbl, vp, sz, vp]
     lib.spect_osem_subiteration.restype = ci

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -747,6 +748,85 @@ int spect_forward(const spect_plan *p, int dtype, const void *f, const void *mu,
                                n, (float *)proj, nullptr, nullptr, nullptr, nullptr, 0.f, ws, st);
   return forward_impl<double>(p, (const double *)f, (const double *)mu, att_table, field, angle_idx,
                               n, (double *)proj, nullptr, nullptr, nullptr, nullptr, 0.0, ws, st);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free forward projection
+// ---------------------------------------------------------------------------
+// The scratch of spect_forward_noalloc: every GPU the library runs on holds
+// its own copy of this array until the process ends.  In host code the name
+// is only a stand-in for the device symbol; the device address comes from
+// hipGetSymbolAddress and is what the launches receive.
+static __device__ __align__(256) unsigned char scratch[SPECT_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the scratch array on `device`, which must be current;
+// looked up on the first call per device and kept.
+int scratch_address(int device, unsigned char **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(SPECT_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(SPECT_E_HIP, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = reinterpret_cast<unsigned char *>(g_scratch_ptr[device]);
+  return SPECT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spect_forward_noalloc(const spect_plan *p, int dtype, const void *f, const void *mu,
+                          const void *att_table, const int32_t *angle_idx, int n, void *proj,
+                          void *stream) {
+  const bool field = (dtype & SPECT_ATT_FIELD) != 0;
+  dtype &= ~SPECT_ATT_FIELD;
+  if (!p) return fail(SPECT_E_ARG, "null plan");
+  if (dtype != SPECT_F32 && dtype != SPECT_F64) return fail(SPECT_E_ARG, "dtype must be F32 or F64");
+  if (dtype == SPECT_F64 && !p->has64) return fail(SPECT_E_ARG, "plan has no float64 tables");
+  if (n <= 0 || n > p->A) return fail(SPECT_E_ARG, "angle count out of range");
+  if (!f || !proj || !angle_idx) return fail(SPECT_E_ARG, "null array argument");
+  if (field && !att_table) return fail(SPECT_E_ARG, "SPECT_ATT_FIELD needs the attenuation field");
+  if (int rc = check_arrays({f, mu, att_table, proj})) return rc;
+  // scratch: FP64 attenuation rows (only when mu comes without a table), then
+  // the chunk partials; every entry is written before it is read
+  const size_t rs = dtype == SPECT_F64 ? 8 : 4;
+  const Geo<float> g = geo<float>(p);
+  const size_t n_jext = g.je1 - g.je0;
+  const size_t rows_b = (mu && !att_table)
+                            ? align256((size_t)n * p->C * n_jext * p->Nz * sizeof(double)) : 0;
+  const size_t part_b = align256((size_t)p->C * n * g.ntiles * (kTJ + 2 * p->H) * p->Nz * rs);
+  if (rows_b + part_b > SPECT_NOALLOC_SCRATCH_BYTES)
+    return fail(SPECT_E_WORKSPACE,
+                "spect_forward_noalloc: " + std::to_string(n) + " angles need " +
+                    std::to_string(rows_b + part_b) + " bytes of scratch, above the limit of " +
+                    std::to_string((size_t)SPECT_NOALLOC_SCRATCH_BYTES) +
+                    " (SPECT_NOALLOC_SCRATCH_BYTES): project fewer angles per call");
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != p->device)
+    return fail(SPECT_E_ARG, "the current device is not the device the plan was created on");
+  unsigned char *base = nullptr;
+  if (int rc = scratch_address(dev, &base)) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  double *rows = reinterpret_cast<double *>(base);
+  if (dtype == SPECT_F32)
+    return forward_core<float>(p, (const float *)f, (const float *)mu, att_table, field, angle_idx,
+                               n, (float *)proj, nullptr, nullptr, nullptr, nullptr, 0.f, rows,
+                               reinterpret_cast<float *>(base + rows_b), st, false);
+  return forward_core<double>(p, (const double *)f, (const double *)mu, att_table, field, angle_idx,
+                              n, (double *)proj, nullptr, nullptr, nullptr, nullptr, 0.0, rows,
+                              reinterpret_cast<double *>(base + rows_b), st, false);
 }
 
 int spect_backward_transpose(const spect_plan *p, int dtype, const void *v, const void *mu,
