@@ -70,8 +70,17 @@ template <typename R> struct Src {
 
 // affine_grid with theta = [[c, -s, 0], [s, c, 0]] followed by grid_sample's
 // unnormalisation (g + 1) * (size - 1) / 2: column ix and row iy (padded px).
+//
+// No floating-point contraction here or in make_src: every coordinate is a
+// rounded value of R, so the fractional part ix - floor(ix) is exact and lies
+// in [0, 1).  If the product (gx + 1) * half were fused into that subtraction
+// (an FMA of the unrounded product minus floor of the rounded one), a sample
+// that lands just below an integer -- as at the quarter-turn angles -- would
+// get a fractional part down to about -6e-6 and a negative bilinear weight,
+// and a non-negative object could project to slightly negative values.
 template <typename R>
 __device__ __forceinline__ void rot_coord(R c, R s, R xn, R yn, R half, R &ix, R &iy) {
+#pragma clang fp contract(off)
   const R gx = c * xn - s * yn;
   const R gy = s * xn + c * yn;
   ix = (gx + R(1)) * half;
@@ -80,6 +89,7 @@ __device__ __forceinline__ void rot_coord(R c, R s, R xn, R yn, R half, R &ix, R
 
 template <typename R>
 __device__ __forceinline__ Src<R> make_src(R ix, R iy, int pad) {
+#pragma clang fp contract(off)
   const R fx = floor(ix), fy = floor(iy);
   Src<R> s;
   s.c0 = int(fx) - pad;
