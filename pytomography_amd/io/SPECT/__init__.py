@@ -1,6 +1,6 @@
-from . import simind
+from . import dicom, simind
 from .shared import (subsample_projections, subsample_metadata,
                      subsample_projections_and_modify_metadata, subsample_amap)
 
-__all__ = ["simind", "subsample_projections", "subsample_metadata",
+__all__ = ["dicom", "simind", "subsample_projections", "subsample_metadata",
            "subsample_projections_and_modify_metadata", "subsample_amap"]

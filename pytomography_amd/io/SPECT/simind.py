@@ -126,8 +126,8 @@ def get_psfmeta_from_header(headerfile: str, min_sigmas=3, lead_attenuation: flo
     """Collimator PSF sigma(d) = sqrt((a d + b)^2 + c^2) from the header's
     hole diameter, thickness, photon energy and intrinsic resolution
     (simind.py:195-233).  The lead attenuation at the photon energy is
-    ``lead_attenuation`` (1/cm) or fitted from a NIST table at
-    ``nist_lead_file``."""
+    ``lead_attenuation`` (1/cm) or fitted from a NIST table: ``nist_lead_file``
+    or, by default, the shipped ``data/NIST_attenuation_data/lead.csv``."""
     h = read_header(headerfile)
     to_cm = 0.1 if float(get_header_value(h, 'program version ', str)[1:2]) >= 8 else 1
     FWHM2sigma = 1 / (2 * np.sqrt(2 * np.log(2)))
@@ -135,10 +135,10 @@ def get_psfmeta_from_header(headerfile: str, min_sigmas=3, lead_attenuation: flo
     hole_length = get_header_value(h, 'Collimator thickness', np.float32) * to_cm
     energy_keV = get_header_value(h, 'Photon Energy', np.float32)
     if lead_attenuation is None:
-        if nist_lead_file is None:
-            raise ValueError("give lead_attenuation (1/cm at the photon energy) or nist_lead_file "
-                             "(a NIST XCOM table for lead)")
         from pytomography_amd.utils.nist_data import get_mu_from_spectrum_interp
+        if nist_lead_file is None:   # the shipped NIST lead table, as the reference (:220)
+            from pytomography_amd.io.SPECT.dicom import DATA_DIR
+            nist_lead_file = os.path.join(DATA_DIR, "NIST_attenuation_data", "lead.csv")
         lead_attenuation = get_mu_from_spectrum_interp(nist_lead_file, energy_keV)
     collimator_slope = hole_diameter / (hole_length - (2 / lead_attenuation)) * 1 / (2 * np.sqrt(2 * np.log(2)))
     intrinsic = get_header_value(h, 'Intrinsic FWHM for the camera', np.float32)

@@ -2,9 +2,9 @@
 and the energy-window scatter estimate (utils/scatter.py:10-99)."""
 from .spatial import (compute_pad_size, compute_pad_size_padded, pad_object, unpad_object,
                       pad_proj, unpad_proj)
-from .misc import get_distance
+from .misc import get_distance, print_collimator_parameters
 from .scatter import compute_EW_scatter, get_smoothed_scatter, get_1d_gaussian_taps
 
 __all__ = ["compute_pad_size", "compute_pad_size_padded", "pad_object", "unpad_object",
-           "pad_proj", "unpad_proj", "get_distance", "compute_EW_scatter",
+           "pad_proj", "unpad_proj", "get_distance", "print_collimator_parameters", "compute_EW_scatter",
            "get_smoothed_scatter", "get_1d_gaussian_taps"]

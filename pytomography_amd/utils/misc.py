@@ -1,4 +1,4 @@
-"""utils/misc.py:21-38 of the reference."""
+"""utils/misc.py:21-66 of the reference."""
 from __future__ import annotations
 
 import numpy as np
@@ -27,3 +27,13 @@ def get_object_nearest_neighbour(object, shifts):
     nb = F.pad(object.unsqueeze(0), 6 * [shift_max], mode="replicate").squeeze(0)
     nb = torch.roll(nb, shifts=[-int(s) for s in shifts], dims=(0, 1, 2))
     return nb[shift_max:-shift_max, shift_max:-shift_max, shift_max:-shift_max]
+
+
+def print_collimator_parameters():
+    """Prints every line of the shipped collimator database, whose records
+    ``io.SPECT.dicom.get_psfmeta_from_scanner_params`` looks up
+    (utils/misc.py:59-66)."""
+    from pytomography_amd.io.SPECT.dicom import COLLIMATOR_FILE
+    with open(COLLIMATOR_FILE) as f:
+        for line in f.readlines():
+            print(line)
