@@ -136,10 +136,12 @@ template <typename R>
 hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
                             const int32_t *idx, int n, int mode, int out_mode, R *out, const R *f,
                             const R *nrm, R delta, R floor_value, const double *Wm,
-                            const double *Pref, R *out_mu, hipStream_t st) {
+                            const double *Pref, R *out_mu, hipStream_t st, size_t w_astride,
+                            size_t w_rstride) {
   constexpr int ZV = sizeof(R) == 4 ? 4 : 2;
   const unsigned blocks = (unsigned)gather_blocks(g);
-  const GatherJob<R> J{g, tb, W, idx, n, out_mode, out, f, nrm, delta, floor_value, Wm, Pref, out_mu};
+  const GatherJob<R> J{g,  tb,      W,      idx,     n,          out_mode, out, f, nrm,
+                       delta, floor_value, Wm, Pref, out_mu, w_astride, w_rstride};
   if (mode == SPECT_BP_REFERENCE)
     bp_gather_kernel<R, ZV, 0><<<blocks, 256, 0, st>>>(J);
   else if (mode == SPECT_BP_ADJOINT)
@@ -205,7 +207,7 @@ hipError_t launch_gather_batch(const GatherJob<R> *jobs, const int *starts, int 
   template hipError_t launch_bp_gather<R>(const Geo<R> &, const Tables<R> &, const R *,        \
                                           const int32_t *, int, int, int, R *, const R *,      \
                                           const R *, R, R, const double *, const double *, R *, \
-                                          hipStream_t);
+                                          hipStream_t, size_t, size_t);
 SPECT_INST(float)
 SPECT_INST(double)
 #undef SPECT_INST

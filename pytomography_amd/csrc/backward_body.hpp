@@ -117,38 +117,46 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
       }
     };
     if constexpr (H > 0) {
-      // ---- A: y blur of the projection with this plane's taps -> U
+      // ---- A: y blur of the projection with this plane's taps -> U, over
+      // the plane's tier of taps [H - HE, H + HE]
+      const int he = ld_uniform(tb.tier + (size_t)rix * g.P + i);
       if (aact) {
         const R *wy = Wt + (size_t)(i - i_lo) * 2 * K + K;
-        R u[HALF_ROWS];
+        with_tier<H>(he, [&](auto he_c) {
+          constexpr int HE = decltype(he_c)::value;
+          R u[HALF_ROWS];
 #pragma unroll
-        for (int r = 0; r < HALF_ROWS; ++r) u[r] = R(0);
+          for (int r = 0; r < HALF_ROWS; ++r) u[r] = R(0);
 #pragma unroll
-        for (int t = 0; t < K; ++t) {
-          const R w = wy[t];
+          for (int t = H - HE; t <= H + HE; ++t) {
+            const R w = wy[t];
 #pragma unroll
-          for (int r = 0; r < HALF_ROWS; ++r) u[r] += w * pw[r + t];
-        }
+            for (int r = 0; r < HALF_ROWS; ++r) u[r] += w * pw[r + t];
+          }
 #pragma unroll
-        for (int r = 0; r < HALF_ROWS; ++r) U[(size_t)(half * HALF_ROWS + r) * ldu + H + za] = u[r];
+          for (int r = 0; r < HALF_ROWS; ++r) U[(size_t)(half * HALF_ROWS + r) * ldu + H + za] = u[r];
+        });
       }
       __syncthreads();
       load_att();
       // ---- B: z blur of the own run
       if (bact) {
         const R *wz = Wt + (size_t)(i - i_lo) * 2 * K;
-        const R *ur = U + (size_t)jl * ldu + z0;  // ur[m] = U(z0 - H + m)
-        R win[RZ + 2 * H];
+        with_tier<H>(he, [&](auto he_c) {
+          constexpr int HE = decltype(he_c)::value;
+          const R *ur = U + (size_t)jl * ldu + z0 + (H - HE);  // ur[m] = U(z0 - HE + m)
+          R win[RZ + 2 * HE];
 #pragma unroll
-        for (int m = 0; m < RZ + 2 * H; ++m) win[m] = ur[m];
+          for (int m = 0; m < RZ + 2 * HE; ++m) win[m] = ur[m];
 #pragma unroll
-        for (int m = 0; m < RZ; ++m) bz[m] = R(0);
+          for (int m = 0; m < RZ; ++m) bz[m] = R(0);
 #pragma unroll
-        for (int t = 0; t < K; ++t) {
-          const R w = wz[t];
+          for (int t = 0; t < 2 * HE + 1; ++t) {
+            const R w = wz[H - HE + t];
 #pragma unroll
-          for (int m = 0; m < RZ; ++m) bz[m] += w * win[m + t];
-        }
+            for (int m = 0; m < RZ; ++m) bz[m] += w * win[m + t];
+          }
+        });
       }
     } else {
       load_att();
@@ -344,6 +352,8 @@ __device__ __forceinline__ void gather_body(const GatherJob<R> &J, size_t gid) {
   const int X = x + g.pad, Y = y + g.pad;
   const int n_jext = g.je1 - g.je0;
   const size_t plane = (size_t)n_jext * Nz;
+  const size_t astride = J.w_astride ? J.w_astride : (size_t)g.P * plane;
+  const size_t rstride = J.w_astride ? J.w_rstride : plane;
   R acc[ZV];
   double accm[MODE == 2 ? ZV : 1];
 #pragma unroll
@@ -370,7 +380,7 @@ __device__ __forceinline__ void gather_body(const GatherJob<R> &J, size_t gid) {
     const int ang = ld_uniform(angle_idx + al);
     if ((unsigned)ang >= (unsigned)g.A) continue;
     const auto cs = ld_uniform(tb.trig + ang);
-    const R *Wa = W + (size_t)al * g.P * plane;
+    const R *Wa = W + (size_t)al * astride;
     if constexpr (MODE == 0) {
       R ix, iy;
       rot_coord<R>(cs.x, -cs.y, linY, linX, g.half, ix, iy);  // theta_b = -theta_f
@@ -392,7 +402,7 @@ __device__ __forceinline__ void gather_body(const GatherJob<R> &J, size_t gid) {
           const int cl = min(max(cc[b], g.je0), g.je1 - 1);
           const R w = ok ? wr[a] * wc[b] : R(0);
           R t[ZV];
-          fetch(Wa + (size_t)r * plane + (size_t)(cl - g.je0) * Nz + z0, t);
+          fetch(Wa + (size_t)r * rstride + (size_t)(cl - g.je0) * Nz + z0, t);
           // ((nw + ne) + sw) + se
 #pragma unroll
           for (int m = 0; m < ZV; ++m) v[m] = (a == 0 && b == 0) ? t[m] * w : v[m] + t[m] * w;
@@ -428,7 +438,7 @@ __device__ __forceinline__ void gather_body(const GatherJob<R> &J, size_t gid) {
           else continue;
           const R w = wr * wc;
           R t[ZV];
-          fetch(Wa + (size_t)i * plane + (size_t)(j - g.je0) * Nz + z0, t);
+          fetch(Wa + (size_t)i * rstride + (size_t)(j - g.je0) * Nz + z0, t);
 #pragma unroll
           for (int m = 0; m < ZV; ++m) v[m] = v[m] + t[m] * w;
           if constexpr (MODE == 2) {

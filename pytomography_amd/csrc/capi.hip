@@ -11,6 +11,7 @@
 
 #include "../../include/spect_hip.h"
 #include "launch.hpp"
+#include "tiers.hpp"
 
 #define SPECT_VERSION 2
 
@@ -34,6 +35,7 @@ struct spect_plan {
   int32_t *all_angles = nullptr;  // 0..A-1
   float *taps_r32 = nullptr, *taps_z32 = nullptr;
   double *taps_r64 = nullptr, *taps_z64 = nullptr;
+  int32_t *tier32 = nullptr, *tier64 = nullptr;  // [n_radii][P] blur tier per plane
 };
 
 namespace {
@@ -95,10 +97,10 @@ std::vector<T> widen_taps(const T *src, int rows, int k, int K, bool unit_if_nul
 
 template <typename R> Tables<R> tables(const spect_plan *p);
 template <> Tables<float> tables<float>(const spect_plan *p) {
-  return {p->trig32, p->lin32, p->radius_index, p->taps_r32, p->taps_z32};
+  return {p->trig32, p->lin32, p->radius_index, p->taps_r32, p->taps_z32, p->tier32};
 }
 template <> Tables<double> tables<double>(const spect_plan *p) {
-  return {p->trig64, p->lin64, p->radius_index, p->taps_r64, p->taps_z64};
+  return {p->trig64, p->lin64, p->radius_index, p->taps_r64, p->taps_z64, p->tier64};
 }
 
 template <typename R> Geo<R> geo(const spect_plan *p) {
@@ -251,6 +253,16 @@ int backward_impl(const spect_plan *p, int mode, int out_mode, const R *proj, co
   int rc = att_rows<R>(p, g, mu, table, field, idx, n, reinterpret_cast<double *>(ws + l.att_off),
                        st, &att, &global, &attf);
   if (rc) return rc;
+  if (!mu && p->H == 0) {
+    // Without attenuation and PSF the field is the projection repeated along
+    // the rows (pad_object 'back_project', utils/spatial.py:76-79): the
+    // gather reads the projection itself (same values, no field pass).
+    const size_t nout = (size_t)(g.jo1 - g.jo0);
+    HIP_TRY(launch_bp_gather<R>(g, tables<R>(p), proj, idx, n, mode, out_mode, out, f, nrm,
+                                R(delta), R(floor_value), nullptr, nullptr, nullptr, st,
+                                nout * g.Nz, 0));
+    return SPECT_OK;
+  }
   R *W = reinterpret_cast<R *>(ws + l.w_off);
   // With the attenuation field the field kernel reads per-plane factors and
   // carries nothing from one chunk to the next, so it runs with its own chunk
@@ -643,11 +655,15 @@ int spect_plan_create(const spect_geom *g, spect_plan **out) {
     auto z32 = widen_taps<float>(g->psf_kz > 0 ? g->taps_z : nullptr, rows, g->psf_kz, K, true);
     if ((rc = upload(&p->taps_r32, r32.data(), r32.size()))) return cleanup(rc);
     if ((rc = upload(&p->taps_z32, z32.data(), z32.size()))) return cleanup(rc);
+    auto t32v = tap_tiers<float>(r32.data(), z32.data(), rows, H, kTierEps32);
+    if ((rc = upload(&p->tier32, t32v.data(), t32v.size()))) return cleanup(rc);
     if (p->has64) {
       auto r64 = widen_taps<double>(g->taps_r64, rows, g->psf_kr, K, false);
       auto z64 = widen_taps<double>(g->psf_kz > 0 ? g->taps_z64 : nullptr, rows, g->psf_kz, K, true);
       if ((rc = upload(&p->taps_r64, r64.data(), r64.size()))) return cleanup(rc);
       if ((rc = upload(&p->taps_z64, z64.data(), z64.size()))) return cleanup(rc);
+      auto t64v = tap_tiers<double>(r64.data(), z64.data(), rows, H, kTierEps64);
+      if ((rc = upload(&p->tier64, t64v.data(), t64v.size()))) return cleanup(rc);
     }
   }
   *out = p;
@@ -666,6 +682,8 @@ int spect_plan_destroy(spect_plan *p) {
   (void)hipFree(p->taps_z32);
   (void)hipFree(p->taps_r64);
   (void)hipFree(p->taps_z64);
+  (void)hipFree(p->tier32);
+  (void)hipFree(p->tier64);
   delete p;
   return SPECT_OK;
 }

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace spect {
 
@@ -46,6 +47,7 @@ template <typename R> struct Tables {
   const int32_t *radius_index;    // [A] -> PSF table row block
   const R *taps_r;                // [n_radii][P][K] taps along r (y), K = 2H+1
   const R *taps_z;                // [n_radii][P][K] taps along z (a unit tap when '1D')
+  const int32_t *tier;            // [n_radii][P] half width each plane's blur runs with
 };
 
 // Geometry of one call.
@@ -294,6 +296,30 @@ __device__ __forceinline__ R att_factor(double S, R x) {
   const R c = R(S);
   const R r = c - x / R(2);
   return att_exp(-r);
+}
+
+// Runs body(std::integral_constant<int, T>) with the blur tier T for a plane
+// whose taps reach h from the centre, in a kernel compiled for half width H:
+// the smallest compiled tier >= h among 2, H - 4k (k >= 1) and H (tiers.hpp);
+// taps beyond T are skipped.  The comparisons are block-uniform.
+template <int H, int T, typename F>
+__device__ __forceinline__ void tier_dispatch(int h, F &body) {
+  if constexpr (T >= H) {
+    body(std::integral_constant<int, H>{});
+  } else {
+    if constexpr (T == 2 || (H - T) % 4 == 0) {
+      if (h <= T) {
+        body(std::integral_constant<int, T>{});
+        return;
+      }
+    }
+    tier_dispatch<H, T + 2>(h, body);
+  }
+}
+template <int H, typename F>
+__device__ __forceinline__ void with_tier(int h, F &&body) {
+  if constexpr (H == 0) body(std::integral_constant<int, 0>{});
+  else tier_dispatch<H, 2>(h, body);
 }
 
 // Does rotated-frame pixel (i, j) sample inside the footprint square?

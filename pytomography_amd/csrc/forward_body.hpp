@@ -289,14 +289,18 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   // (VEC is a compile-time constant inside the phase bodies): a per-load
   // runtime branch would make the compiler wait for every outstanding load at
   // each branch merge and serialise the independent gathers.
+  // (a block-uniform 64-bit plane base plus 32-bit lane offsets: fewer
+  // vector registers than a 64-bit address per pass)
+  const uint32_t af_col0 = (uint32_t)((j0 - g.je0 + aco) * Nz + za);
   auto issue_v = [&](int ip, auto vec_c) {
     constexpr bool VEC = decltype(vec_c)::value;
+    const R *base = attf + ((size_t)ang * g.P + ip) * ((size_t)n_jext * Nz);
 #pragma unroll
     for (int p = 0; p < NPA; ++p) {
       const int jj = aco + p * cpp;
       if (jj < tj)
-        load_zv<R, ZV>(attf + (((size_t)ang * g.P + ip) * n_jext + (j0 - g.je0 + jj)) * Nz, za, Nz,
-                       VEC, Ar[PF ? p : 0]);
+        load_zv<R, ZV>(base + (af_col0 + (uint32_t)(p * cpp * Nz)), 0, Nz - za, VEC,
+                       Ar[PF ? p : 0]);
     }
   };
   auto issue = [&](int ip, int bb) {
@@ -311,6 +315,8 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   while (i >= 0) {
     const int inext = rem ? i_lo + 63 - __clzll(rem) : -1;
     if (inext >= 0) rem &= ~(1ull << (inext - i_lo));
+    // the plane's blur tier (a scalar load, requested before the gather)
+    const int he = H > 0 ? ld_uniform(tb.tier + (size_t)rix * g.P + i) : 0;
     // ---- A: V = a * f_rot, ZV z per lane, NPA columns per thread
     auto phase_a = [&](auto vec_c) {
       constexpr bool VEC = decltype(vec_c)::value;
@@ -447,7 +453,8 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
       if constexpr (PF) {
         if (inext >= 0) issue(inext, b ^ 1);
       }
-      // ---- B: z blur (correlation, zero padded) of the own run
+      // ---- B: z blur (correlation, zero padded) of the own run, over the
+      // plane's tier of taps [H - HE, H + HE]
       const R *wz = Wt + (size_t)(i - i_lo) * 2 * K;
       if (ab) {
         R win[RZ + 2 * H];
@@ -457,12 +464,15 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
         R out[RZ];
 #pragma unroll
         for (int m = 0; m < RZ; ++m) out[m] = R(0);
+        with_tier<H>(he, [&](auto he_c) {
+          constexpr int HE = decltype(he_c)::value;
 #pragma unroll
-        for (int t = 0; t < K; ++t) {
-          const R w = wz[t];
+          for (int t = H - HE; t <= H + HE; ++t) {
+            const R w = wz[t];
 #pragma unroll
-          for (int m = 0; m < RZ; ++m) out[m] += w * win[m + t];
-        }
+            for (int m = 0; m < RZ; ++m) out[m] += w * win[m + t];
+          }
+        });
 #pragma unroll
         for (int m = 0; m < RZ; ++m) Vz[(size_t)jl * ldv + z0 + m] = out[m];
       }
@@ -479,17 +489,20 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
           R v[kTJ];
 #pragma unroll
           for (int jv = jv0; jv < jv1; ++jv) v[jv] = Vz[(size_t)jv * ldv + zc];
-          // out[jo] += w[t] * in[jo + t - 2H] (tile-local); t ascending is jv ascending
+          with_tier<H>(he, [&](auto he_c) {
+            constexpr int HE = decltype(he_c)::value;
+            // out[jo] += w[t] * in[jo + t - 2H] (tile-local); t ascending is jv ascending
 #pragma unroll
-          for (int t = 0; t < K; ++t) {
-            const R w = wy[t];
+            for (int t = H - HE; t <= H + HE; ++t) {
+              const R w = wy[t];
 #pragma unroll
-            for (int q = 0; q < JH; ++q) {
-              const int jo = jlo + q;
-              const int jv = jo + t - 2 * H;
-              if (jo < jhi && jv >= jv0 && jv < jv1) acc[q] += w * v[jv];
+              for (int q = 0; q < JH; ++q) {
+                const int jo = jlo + q;
+                const int jv = jo + t - 2 * H;
+                if (jo < jhi && jv >= jv0 && jv < jv1) acc[q] += w * v[jv];
+              }
             }
-          }
+          });
         };
         switch (half) {
           case 0: scatter(std::integral_constant<int, 0>{}); break;

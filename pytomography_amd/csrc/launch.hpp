@@ -69,6 +69,12 @@ template <typename R> struct GatherJob {
   const double *Wm;
   const double *Pref;
   R *out_mu;
+  // Strides (elements) of W between angles and between rows; 0 selects the
+  // field layout [n][P][je1-je0][Nz].  Without attenuation and PSF the field
+  // is the projection repeated along the rows, so the gather can read the
+  // projection [n][jo1-jo0][Nz] itself: angle stride (jo1-jo0)*Nz, row stride 0.
+  size_t w_astride;
+  size_t w_rstride;
 };
 
 // Blocks per job of each kernel (the batched launchers sum these).
@@ -146,7 +152,8 @@ template <typename R>
 hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
                             const int32_t *angle_idx, int n, int mode, int out_mode, R *out,
                             const R *f, const R *nrm, R delta, R floor_value, const double *Wm,
-                            const double *Pref, R *out_mu, hipStream_t st);
+                            const double *Pref, R *out_mu, hipStream_t st,
+                            size_t w_astride = 0, size_t w_rstride = 0);
 
 // Elementwise and stencil kernels (misc.hip).
 template <typename R>

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "tiers.hpp"
 
 thread_local cpu_dim3 threadIdx;
 cpu_dim3 blockDim;
@@ -78,10 +79,10 @@ struct Problem {
   spect::Geo<float> g;
   std::vector<float2> trig;
   std::vector<float> lin, taps_r, taps_z, mu;
-  std::vector<int32_t> ridx, sel;
+  std::vector<int32_t> ridx, sel, tier;
   spect::Tables<float> tables() const {
     return {trig.data(), lin.data(), ridx.data(), H > 0 ? taps_r.data() : nullptr,
-            H > 0 ? taps_z.data() : nullptr};
+            H > 0 ? taps_z.data() : nullptr, H > 0 ? tier.data() : nullptr};
   }
 };
 
@@ -135,6 +136,7 @@ inline Problem load_problem(const std::string &dir) {
     std::vector<float> tz;
     if (p.kz > 0) tz = read_file<float>(dir + "/taps_z.bin", (size_t)rows * p.kz);
     p.taps_z = widen(tz, rows, p.kz, K, true);
+    p.tier = spect::tap_tiers<float>(p.taps_r.data(), p.taps_z.data(), rows, p.H, spect::kTierEps32);
   }
   // theta_f = -(270 - beta) in float32, radians as float32 deg2rad
   p.trig.resize(p.A);
