@@ -88,6 +88,10 @@ typedef struct spect_geom {
   int32_t chunk_planes;    /* planes per work chunk (0: default, 16 planes, and 24 for the
                               back projection's field kernel in attenuation-field mode;
                               a positive value applies to every kernel) */
+  int32_t sched_rounds;    /* forward without attenuation or with the attenuation field:
+                              its visited planes are split evenly over sched_rounds x the
+                              blocks the GPU holds at once (0: 1 round); -1: uniform
+                              chunks of chunk_planes planes instead */
 } spect_geom;
 
 const char *spect_last_error(void);

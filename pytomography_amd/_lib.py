@@ -55,6 +55,7 @@ class SpectGeom(ctypes.Structure):
         ("taps_r64", ctypes.POINTER(ctypes.c_double)),
         ("taps_z64", ctypes.POINTER(ctypes.c_double)),
         ("chunk_planes", ctypes.c_int32),
+        ("sched_rounds", ctypes.c_int32),
     ]
 
 

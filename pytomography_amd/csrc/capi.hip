@@ -11,6 +11,7 @@
 
 #include "../../include/spect_hip.h"
 #include "launch.hpp"
+#include "activity.hpp"
 #include "tiers.hpp"
 
 #define SPECT_VERSION 2
@@ -22,6 +23,7 @@ struct spect_plan {
   double dx, dy, dz;
   int H;                 // PSF half width used by the kernels (0: none)
   int L, C;              // planes per chunk, chunk count
+  int rounds;            // rounds of the forward's balanced schedule (-1: uniform chunks)
   int L_field;           // planes per chunk of the back projection's field kernel in
                          // attenuation-field mode (no state carried across chunks)
   int n_radii;
@@ -36,6 +38,8 @@ struct spect_plan {
   float *taps_r32 = nullptr, *taps_z32 = nullptr;
   double *taps_r64 = nullptr, *taps_z64 = nullptr;
   int32_t *tier32 = nullptr, *tier64 = nullptr;  // [n_radii][P] blur tier per plane
+  unsigned long long *act = nullptr;  // [A][ntiles][C] visited planes per forward block
+  int2 *ranges = nullptr;             // [A][ntiles] visited plane range per (angle, tile)
 };
 
 namespace {
@@ -97,10 +101,12 @@ std::vector<T> widen_taps(const T *src, int rows, int k, int K, bool unit_if_nul
 
 template <typename R> Tables<R> tables(const spect_plan *p);
 template <> Tables<float> tables<float>(const spect_plan *p) {
-  return {p->trig32, p->lin32, p->radius_index, p->taps_r32, p->taps_z32, p->tier32};
+  return {p->trig32, p->lin32, p->radius_index, p->taps_r32, p->taps_z32, p->tier32, p->act,
+          p->ranges};
 }
 template <> Tables<double> tables<double>(const spect_plan *p) {
-  return {p->trig64, p->lin64, p->radius_index, p->taps_r64, p->taps_z64, p->tier64};
+  return {p->trig64, p->lin64, p->radius_index, p->taps_r64, p->taps_z64, p->tier64, p->act,
+          p->ranges};
 }
 
 template <typename R> Geo<R> geo(const spect_plan *p) {
@@ -131,13 +137,37 @@ struct Ws {
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// The forward's partials for n angles: room for the chunk layout
+// [C][n][ntiles] and for the balanced schedule's slots (one per block) of
+// (kTJ + 2H) x Nz values each, followed by the schedule's ints.
+struct FwdPart {
+  size_t sched_off, total;  // relative to the start of the partials
+  int sched_blocks;         // blocks of the scheduled forward (0: not runnable)
+};
+
+FwdPart fwd_part_layout(const spect_plan *p, int dtype, int n) {
+  const size_t rs = dtype == SPECT_F64 ? 8 : 4;
+  const Geo<float> g = geo<float>(p);
+  const size_t JO = kTJ + 2 * p->H;
+  const size_t npairs = (size_t)n * g.ntiles;
+  FwdPart l;
+  l.sched_blocks = p->rounds < 0 ? 0
+                   : p->rounds * (dtype == SPECT_F64 ? forward_sched_blocks<double>(geo<double>(p))
+                                                     : forward_sched_blocks<float>(g));
+  const size_t chunk = (size_t)p->C * npairs * JO * p->Nz * rs;
+  const size_t sched =
+      (size_t)sched_grid_blocks(n, g.ntiles, p->P, l.sched_blocks) * JO * p->Nz * rs;
+  l.sched_off = align256(std::max(chunk, sched));
+  l.total = l.sched_off + align256((2 + npairs + 1) * sizeof(int));
+  return l;
+}
+
 Ws ws_layout(const spect_plan *p, int dtype, int n) {
   const size_t rs = dtype == SPECT_F64 ? 8 : 4;
   const Geo<float> g = geo<float>(p);
   const size_t n_jext = g.je1 - g.je0;
-  const size_t JO = kTJ + 2 * p->H;
   const size_t att = align256((size_t)n * p->C * n_jext * p->Nz * sizeof(double));
-  const size_t part = align256((size_t)p->C * n * g.ntiles * JO * p->Nz * rs);
+  const size_t part = fwd_part_layout(p, dtype, n).total;
   const size_t w = align256((size_t)n * p->P * n_jext * p->Nz * rs);
   const size_t ratio = align256((size_t)n * p->N * p->Nz * rs);
   Ws l;
@@ -225,6 +255,15 @@ int forward_core(const spect_plan *p, const R *f, const R *mu, const void *table
   const R *attf;
   int rc = att_rows<R>(p, g, mu, table, field, idx, n, rows, st, &att, &global, &attf);
   if (rc) return rc;
+  const FwdPart pl = fwd_part_layout(p, sizeof(R) == 8 ? SPECT_F64 : SPECT_F32, n);
+  if (!trb && (attf || !mu) && pl.sched_blocks > 0) {
+    // the balanced schedule: every block carries the same number of planes
+    int *sched = reinterpret_cast<int *>(reinterpret_cast<unsigned char *>(part) + pl.sched_off);
+    HIP_TRY(launch_forward_sched<R>(g, tables<R>(p), f, attf, idx, n, part, sched,
+                                    pl.sched_blocks, st));
+    HIP_TRY(launch_reduce_sched<R>(g, part, n, sched, proj, gm, s, pred, ratio, delta, st));
+    return SPECT_OK;
+  }
   HIP_TRY(launch_forward<R>(g, tables<R>(p), f, mu, att, global, attf, idx, n, part, trb, st));
   HIP_TRY(launch_forward_reduce<R>(g, part, n, proj, gm, s, pred, ratio, delta, st));
   return SPECT_OK;
@@ -579,6 +618,8 @@ int spect_plan_create(const spect_geom *g, spect_plan **out) {
   if (g->nz > 256) return fail(SPECT_E_ARG, "nz > 256 is not supported by the tile kernels");
   if (g->chunk_planes < 0 || g->chunk_planes > 64)
     return fail(SPECT_E_ARG, "chunk_planes must be 0 (default) or 1..64");
+  if (g->sched_rounds < -1 || g->sched_rounds > 16)
+    return fail(SPECT_E_ARG, "sched_rounds must be -1, 0 (default) or 1..16");
   if (g->n_angles <= 0 || !g->angles_deg) return fail(SPECT_E_ARG, "angles required");
   if (!(g->dx > 0) || !(g->dz > 0)) return fail(SPECT_E_ARG, "voxel sizes must be positive");
   if (g->psf_kr < 0 || g->psf_kz < 0 || (g->psf_kr > 0 && g->psf_kr % 2 == 0) ||
@@ -609,6 +650,7 @@ int spect_plan_create(const spect_geom *g, spect_plan **out) {
   // Default: 24-plane field chunks (fewer blocks, each keeping its projection
   // window for more planes); an explicit chunk_planes applies to every kernel.
   p->L_field = g->chunk_planes > 0 ? g->chunk_planes : 24;
+  p->rounds = g->sched_rounds == 0 ? 1 : g->sched_rounds;
   p->n_radii = g->psf_kr > 0 ? g->n_radii : 1;
   p->has64 = g->angles_deg64 != nullptr && (g->psf_kr == 0 || g->taps_r64 != nullptr);
   (void)hipGetDevice(&p->device);
@@ -646,6 +688,24 @@ int spect_plan_create(const spect_geom *g, spect_plan **out) {
   auto l64 = host_linspace<double>(p->P);
   if ((rc = upload(&p->lin32, l32.data(), l32.size()))) return cleanup(rc);
   if ((rc = upload(&p->lin64, l64.data(), l64.size()))) return cleanup(rc);
+  {
+    // planes each forward block visits (activity.hpp), from the float64 geometry
+    const Geo<double> g = geo<double>(p);
+    std::vector<double> cv(p->A), sv(p->A);
+    for (int a = 0; a < p->A; ++a) {
+      cv[a] = t64[a].x;
+      sv[a] = t64[a].y;
+    }
+    std::vector<int> rng;
+    auto act = forward_activity(cv.data(), sv.data(), p->A, l64.data(), p->P, g.je0, g.je1, p->L,
+                                p->C, kTJ, (double)g.lo, (double)g.hi, &rng);
+    static_assert(sizeof(unsigned long long) == sizeof(uint64_t), "64-bit masks");
+    if ((rc = upload(&p->act, reinterpret_cast<const unsigned long long *>(act.data()), act.size())))
+      return cleanup(rc);
+    std::vector<int2> r2(rng.size() / 2);
+    for (size_t k = 0; k < r2.size(); ++k) r2[k] = make_int2(rng[2 * k], rng[2 * k + 1]);
+    if ((rc = upload(&p->ranges, r2.data(), r2.size()))) return cleanup(rc);
+  }
   if ((rc = upload(&p->radius_index, ridx.data(), ridx.size()))) return cleanup(rc);
   if ((rc = upload(&p->all_angles, all.data(), all.size()))) return cleanup(rc);
   if (H > 0) {
@@ -684,6 +744,8 @@ int spect_plan_destroy(spect_plan *p) {
   (void)hipFree(p->taps_z64);
   (void)hipFree(p->tier32);
   (void)hipFree(p->tier64);
+  (void)hipFree(p->act);
+  (void)hipFree(p->ranges);
   delete p;
   return SPECT_OK;
 }
@@ -823,7 +885,7 @@ int spect_forward_noalloc(const spect_plan *p, int dtype, const void *f, const v
   const size_t n_jext = g.je1 - g.je0;
   const size_t rows_b = (mu && !att_table)
                             ? align256((size_t)n * p->C * n_jext * p->Nz * sizeof(double)) : 0;
-  const size_t part_b = align256((size_t)p->C * n * g.ntiles * (kTJ + 2 * p->H) * p->Nz * rs);
+  const size_t part_b = fwd_part_layout(p, dtype, n).total;
   if (rows_b + part_b > SPECT_NOALLOC_SCRATCH_BYTES)
     return fail(SPECT_E_WORKSPACE,
                 "spect_forward_noalloc: " + std::to_string(n) + " angles need " +

@@ -21,6 +21,16 @@ namespace spect {
 
 constexpr int kTJ = 32;  // rotated-frame columns per tile (forward and back projection)
 
+// Most planes in one piece of the forward's balanced schedule.
+constexpr int kPieceMax = 32;
+
+// Blocks the scheduled forward launches for n angles: the resident blocks,
+// or more when even kPieceMax-plane pieces outnumber them (large calls).
+__host__ __device__ inline int sched_grid_blocks(int n, int ntiles, int P, int resident) {
+  const int most = n * ntiles * ((P + kPieceMax - 1) / kPieceMax);
+  return most > resident ? most : resident;
+}
+
 template <typename R> struct V2;
 template <> struct V2<float> { using T = float2; };
 template <> struct V2<double> { using T = double2; };
@@ -48,6 +58,12 @@ template <typename R> struct Tables {
   const R *taps_r;                // [n_radii][P][K] taps along r (y), K = 2H+1
   const R *taps_z;                // [n_radii][P][K] taps along z (a unit tap when '1D')
   const int32_t *tier;            // [n_radii][P] half width each plane's blur runs with
+  // [A][ntiles][C] planes of each forward block that can touch the object
+  // (bit l: plane c * L + l), or null: the forward kernel tests them itself
+  const unsigned long long *act;
+  // [A][ntiles] (i_min, i_max) of the planes each (angle, tile) visits
+  // (i_max < i_min: none), the balanced schedule's input
+  const int2 *ranges;
 };
 
 // Geometry of one call.

@@ -19,6 +19,12 @@
 // Planes where no column of the tile samples the object are skipped.  Each
 // block writes its kTJ + 2H output columns; a reduce kernel adds chunks and
 // overlapping tiles in a fixed order (deterministic, no atomics).
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <utility>
+
 #include "forward_body.hpp"
 
 namespace spect {
@@ -53,8 +59,30 @@ template <typename R, int RZ, int QS, int H, bool TRB, bool AF>
 __global__ __launch_bounds__(512) void fwd_kernel(FwdJob<R> J) {
   extern __shared__ __align__(16) unsigned char smem[];
   // tiles and chunks are taken centre-out: the heaviest blocks are dispatched first
-  fwd_body<R, RZ, QS, H, TRB, AF>(J, blockIdx.x, centre_out(blockIdx.y, J.g.ntiles),
-                                  centre_out(blockIdx.z, J.g.C), smem);
+  const int jt = centre_out(blockIdx.y, J.g.ntiles);
+  fwd_body<R, RZ, QS, H, TRB, AF>(J, blockIdx.x, jt,
+                                  chunk_piece(J.g, J.n, blockIdx.x, jt, centre_out(blockIdx.z, J.g.C)),
+                                  smem);
+}
+
+// The balanced schedule of a call (one block) and the forward blocks that
+// walk it (schedule_body / fwd_sched_body).
+template <typename R>
+__global__ __launch_bounds__(1024) void schedule_kernel(Geo<R> g, const int32_t *__restrict__ idx,
+                                                        const int2 *__restrict__ ranges, int n,
+                                                        int nblocks, int *__restrict__ sched) {
+  schedule_body<R>(g, idx, ranges, n, nblocks, sched);
+}
+
+template <typename R, int RZ, int QS, int H, bool AF>
+__global__ __launch_bounds__(512) void fwd_sched_kernel(FwdJob<R> J, const int *__restrict__ sched) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  fwd_sched_body<R, RZ, QS, H, AF>(J, J.tb.ranges, sched, blockIdx.x, smem);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void reduce_sched_kernel(ReduceJob<R> J, const int *__restrict__ sched) {
+  reduce_sched_body<R>(J, sched, (size_t)blockIdx.x * 256 + threadIdx.x);
 }
 
 // Batched: block b belongs to job find_job(b); within a job the blocks run
@@ -69,7 +97,8 @@ __global__ __launch_bounds__(512) void fwd_batch_kernel(const FwdJob<R> *__restr
   const int local = b - ld_uniform(starts + j);
   const int per_chunk = J.n * J.g.ntiles;
   const int c = centre_out(local / per_chunk, J.g.C), r = local % per_chunk;
-  fwd_body<R, RZ, QS, H, false, AF>(J, r % J.n, r / J.n, c, smem);
+  const int al = r % J.n, jt = r / J.n;
+  fwd_body<R, RZ, QS, H, false, AF>(J, al, jt, chunk_piece(J.g, J.n, al, jt, c), smem);
 }
 
 template <typename R>
@@ -95,8 +124,7 @@ hipError_t launch_fwd_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const 
   const int bt = tile_block_threads(g.Nz, RZ, QS);
   if (bt > 512) return hipErrorInvalidConfiguration;  // __launch_bounds__(512)
   if (!phase_a_fits<R, QS>(bt, g.Nz)) return hipErrorInvalidConfiguration;
-  const int runs = (g.Nz + RZ - 1) / RZ;
-  const size_t smem = (2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)g.L * 2 * (2 * H + 1)) * sizeof(R);
+  const size_t smem = fwd_smem_bytes<R, RZ, H>(g.Nz, g.L);
   auto kern = attf ? (trb ? fwd_kernel<R, RZ, QS, H, true, true> : fwd_kernel<R, RZ, QS, H, false, true>)
                    : (trb ? fwd_kernel<R, RZ, QS, H, true, false> : fwd_kernel<R, RZ, QS, H, false, false>);
   if (smem > 64 * 1024) {
@@ -117,8 +145,7 @@ hipError_t launch_fwd_batch_h(const FwdJob<R> *jobs, const int *starts, int njob
   const int bt = tile_block_threads(Nz, RZ, QS);
   if (bt > 512) return hipErrorInvalidConfiguration;  // __launch_bounds__(512)
   if (!phase_a_fits<R, QS>(bt, Nz)) return hipErrorInvalidConfiguration;
-  const int runs = (Nz + RZ - 1) / RZ;
-  const size_t smem = (2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)max_L * 2 * (2 * H + 1)) * sizeof(R);
+  const size_t smem = fwd_smem_bytes<R, RZ, H>(Nz, max_L);
   auto kern = af ? fwd_batch_kernel<R, RZ, QS, H, true> : fwd_batch_kernel<R, RZ, QS, H, false>;
   if (smem > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
@@ -129,7 +156,100 @@ hipError_t launch_fwd_batch_h(const FwdJob<R> *jobs, const int *starts, int njob
   return hipGetLastError();
 }
 
+// Blocks of fwd_sched_kernel resident on the device at once (occupancy x
+// CUs), cached per kernel instance, z extent and device.
+template <typename R, int H, bool AF>
+int sched_resident_blocks(int Nz, int L) {
+  constexpr int RZ = FwdTile<R>::RZ, QS = FwdTile<R>::QS;
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find({dev, Nz, L});
+  if (it != cache.end()) return it->second;
+  const int bt = tile_block_threads(Nz, RZ, QS);
+  const size_t smem = fwd_smem_bytes<R, RZ, H>(Nz, L);
+  auto kern = fwd_sched_kernel<R, RZ, QS, H, AF>;
+  if (smem > 64 * 1024 &&
+      hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)smem) != hipSuccess)
+    return 0;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, bt, smem) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const int nb = per_cu > 0 && cus > 0 ? per_cu * cus : 0;
+  cache[{dev, Nz, L}] = nb;
+  return nb;
+}
+
+template <typename R, int H>
+int sched_blocks_h(int Nz, int L, bool af) {
+  return af ? sched_resident_blocks<R, H, true>(Nz, L) : sched_resident_blocks<R, H, false>(Nz, L);
+}
+
+template <typename R, int H>
+hipError_t launch_fwd_sched_h(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *attf,
+                              const int32_t *idx, int n, R *partial, int *sched, int nb,
+                              hipStream_t st) {
+  constexpr int RZ = FwdTile<R>::RZ, QS = FwdTile<R>::QS;
+  const int bt = tile_block_threads(g.Nz, RZ, QS);
+  if (bt > 512) return hipErrorInvalidConfiguration;  // __launch_bounds__(512)
+  if (!phase_a_fits<R, QS>(bt, g.Nz)) return hipErrorInvalidConfiguration;
+  if (nb <= 0) return hipErrorInvalidConfiguration;
+  const int grid = sched_grid_blocks(n, g.ntiles, g.P, nb);
+  const size_t smem = fwd_smem_bytes<R, RZ, H>(g.Nz, g.L);
+  auto kern = attf ? fwd_sched_kernel<R, RZ, QS, H, true> : fwd_sched_kernel<R, RZ, QS, H, false>;
+  if (smem > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+  }
+  const int npairs = n * g.ntiles;
+  const int sbt = npairs >= 1024 ? 1024 : ((npairs + 63) / 64) * 64;
+  schedule_kernel<R><<<1, sbt, 0, st>>>(g, idx, tb.ranges, n, nb, sched);
+  FwdJob<R> J{g, tb, f, nullptr, nullptr, 0, attf, idx, n, partial};
+  kern<<<grid, bt, smem, st>>>(J, sched);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+template <typename R>
+int forward_sched_blocks(const Geo<R> &g) {
+#define SPECT_FS_CASE(HH) \
+  case HH: return std::max(sched_blocks_h<R, HH>(g.Nz, g.L, false), sched_blocks_h<R, HH>(g.Nz, g.L, true));
+  switch (g.H) {
+    SPECT_FS_CASE(0) SPECT_FS_CASE(2) SPECT_FS_CASE(4) SPECT_FS_CASE(6) SPECT_FS_CASE(8)
+    SPECT_FS_CASE(10) SPECT_FS_CASE(12) SPECT_FS_CASE(14) SPECT_FS_CASE(16) SPECT_FS_CASE(18)
+    SPECT_FS_CASE(20)
+    default: return 0;
+  }
+#undef SPECT_FS_CASE
+}
+
+template <typename R>
+hipError_t launch_forward_sched(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *attf,
+                                const int32_t *idx, int n, R *partial, int *sched, int blocks,
+                                hipStream_t st) {
+#define SPECT_FSL_CASE(HH) \
+  case HH: return launch_fwd_sched_h<R, HH>(g, tb, f, attf, idx, n, partial, sched, blocks, st);
+  switch (g.H) {
+    SPECT_FSL_CASE(0) SPECT_FSL_CASE(2) SPECT_FSL_CASE(4) SPECT_FSL_CASE(6) SPECT_FSL_CASE(8)
+    SPECT_FSL_CASE(10) SPECT_FSL_CASE(12) SPECT_FSL_CASE(14) SPECT_FSL_CASE(16)
+    SPECT_FSL_CASE(18) SPECT_FSL_CASE(20)
+    default: return hipErrorInvalidValue;
+  }
+#undef SPECT_FSL_CASE
+}
+
+template <typename R>
+hipError_t launch_reduce_sched(const Geo<R> &g, const R *partial, int n, const int *sched, R *proj,
+                               const R *gm, const R *s, R *pred, R *ratio, R delta, hipStream_t st) {
+  ReduceJob<R> J{g, partial, n, proj, gm, s, pred, ratio, delta};
+  reduce_sched_kernel<R><<<reduce_blocks(g, n), 256, 0, st>>>(J, sched);
+  return hipGetLastError();
+}
 
 template <typename R>
 hipError_t launch_att_table(const Geo<R> &g, const Tables<R> &tb, const R *mu,
@@ -210,7 +330,13 @@ hipError_t launch_reduce_batch(const ReduceJob<R> *jobs, const int *starts, int 
   template hipError_t launch_forward_batch<R>(const FwdJob<R> *, const int *, int, int, int,    \
                                               int, int, bool, hipStream_t);                    \
   template hipError_t launch_reduce_batch<R>(const ReduceJob<R> *, const int *, int, int,       \
-                                             hipStream_t);
+                                             hipStream_t);                                     \
+  template int forward_sched_blocks<R>(const Geo<R> &);                                        \
+  template hipError_t launch_forward_sched<R>(const Geo<R> &, const Tables<R> &, const R *,     \
+                                              const R *, const int32_t *, int, R *, int *, int, \
+                                              hipStream_t);                                    \
+  template hipError_t launch_reduce_sched<R>(const Geo<R> &, const R *, int, const int *, R *,  \
+                                             const R *, const R *, R *, R *, R, hipStream_t);
 SPECT_INST(float)
 SPECT_INST(double)
 #undef SPECT_INST

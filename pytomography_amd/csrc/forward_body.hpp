@@ -98,27 +98,43 @@ __device__ __forceinline__ void att_field_body(const Geo<R> &g, const Tables<R> 
       src[tid] = make_src<R>(ix, iy, g.pad);
       inside = in_footprint<R>(ix, iy, g);
     }
-    if (__syncthreads_or(inside)) {
-      if (mine) {
+    // Every plane gets its factors, also where the tile samples no object
+    // (mu is zero there, so S carries over): the projectors may visit a
+    // plane a conservative activity test keeps.
+    const bool any = __syncthreads_or(inside);
+    if (mine) {
+      R x[RZ];
+      if (any) {
         const Taps4<R> t = taps4<R>(src[jl], g.N, g.Nz);
-        R x[RZ];
         gather_run<R, RZ>(mu, t, z0, g.Nz, x);
-        R *dst = field + (((size_t)al * g.P + i) * n_jext + (j0 - g.je0 + jl)) * g.Nz + z0;
+      } else {
 #pragma unroll
-        for (int m = 0; m < RZ; ++m) {
-          const R xm = x[m] * g.dx;
-          S[m] += (double)xm;
-          if (z0 + m < g.Nz) dst[m] = att_factor<R>(S[m], xm);
-        }
+        for (int m = 0; m < RZ; ++m) x[m] = R(0);
       }
-      __syncthreads();  // src is rewritten by the next plane
+      R *dst = field + (((size_t)al * g.P + i) * n_jext + (j0 - g.je0 + jl)) * g.Nz + z0;
+#pragma unroll
+      for (int m = 0; m < RZ; ++m) {
+        const R xm = x[m] * g.dx;
+        S[m] += (double)xm;
+        if (z0 + m < g.Nz) dst[m] = att_factor<R>(S[m], xm);
+      }
     }
+    __syncthreads();  // src is rewritten by the next plane
   }
 }
 
 // LDS rows: H zeros | runs*RZ values | H zeros | 1 pad (odd stride).
 template <int RZ, int H> __device__ __forceinline__ int lds_stride(int runs) {
   return runs * RZ + 2 * H + 1;
+}
+
+// Dynamic LDS of fwd_body: V and Vz ([kTJ][stride] each) and the taps of the
+// piece's planes ([planes][2][2H+1], planes = the larger of the chunk length
+// and kPieceMax).
+template <typename R, int RZ, int H> inline size_t fwd_smem_bytes(int Nz, int L) {
+  const int runs = (Nz + RZ - 1) / RZ;
+  const int planes = L > kPieceMax ? L : kPieceMax;
+  return (2 * (size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)planes * 2 * (2 * H + 1)) * sizeof(R);
 }
 
 // TRB: instead of the forward bilinear gather of f, the transpose of the
@@ -132,8 +148,30 @@ template <int RZ, int H> __device__ __forceinline__ int lds_stride(int runs) {
 constexpr int kTrbSearch = 4;
 constexpr int kTrbMax = kTrbSearch * kTrbSearch;
 
+// The planes one call of fwd_body covers and where its result goes.
+struct FwdPiece {
+  int i_lo, i_hi;  // planes [i_lo, i_hi) (at most 64), walked from i_hi - 1 down
+  int c;           // chunk of the FP64 suffix-table rows the march starts from
+  int ranged;      // 1: every plane of the range is visited; 0: the chunk's
+                   // activity mask (table or in-kernel test) selects them
+  size_t slot;     // partial + slot * (kTJ + 2H) * Nz receives the tile's sums
+};
+
+// Chunk c of the uniform schedule: planes [c L, c L + L), partial slot
+// (c, al, jt) of the [C][n][ntiles] layout.
+template <typename R>
+__device__ __forceinline__ FwdPiece chunk_piece(const Geo<R> &g, int n, int al, int jt, int c) {
+  FwdPiece pc;
+  pc.i_lo = c * g.L;
+  pc.i_hi = min(g.P, pc.i_lo + g.L);
+  pc.c = c;
+  pc.ranged = 0;
+  pc.slot = ((size_t)c * n + al) * g.ntiles + jt;
+  return pc;
+}
+
 template <typename R, int RZ, int QS, int H, bool TRB, bool AF>
-__device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int c,
+__device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, const FwdPiece &pc,
                                          unsigned char *smem) {
   constexpr int K = 2 * H + 1;
   constexpr int JO = kTJ + 2 * H;      // output columns touched by the tile
@@ -154,7 +192,7 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   const R *__restrict__ attf = J.attf;  // AF: precomputed attenuation factors
   const int32_t *__restrict__ angle_idx = J.angle_idx;
   R *__restrict__ partial = J.partial;
-  const int n_call = J.n;
+  const int c = pc.c;
   const int ang = ld_uniform(angle_idx + al);
   if ((unsigned)ang >= (unsigned)g.A) return;
   const int tid = threadIdx.x;
@@ -187,9 +225,29 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   const auto cs = ld_uniform(tb.trig + ang);
   const bool has_mu = mu != nullptr;
 
-  // V and Vz start zeroed: halo pads, columns >= tj and z slots >= Nz are
-  // never written afterwards
-  for (int q = tid; q < 2 * kTJ * ldv; q += blockDim.x) V[q] = R(0);
+  // The LDS entries the phases read but never write start at zero: V's
+  // halo pads, V's z slots past Nz, and whole rows of both buffers past the
+  // tile's last column (phase A writes columns < tj, z < Nz; phase B writes
+  // Vz's data slots of every row)
+  {
+    const int pads = 2 * H + 1;  // per row: [0, H) and [H + runs*RZ, ldv)
+    for (int q = tid; q < kTJ * pads; q += blockDim.x) {
+      const int row = q / pads, k = q - row * pads;
+      V[(size_t)row * ldv + (k < H ? k : runs * RZ + k)] = R(0);
+    }
+    const int tailz = runs * RZ - Nz;
+    if (tailz > 0)
+      for (int q = tid; q < kTJ * tailz; q += blockDim.x) {
+        const int row = q / tailz, k = q - row * tailz;
+        V[(size_t)row * ldv + H + Nz + k] = R(0);
+        if constexpr (H == 0) Vz[(size_t)row * ldv + Nz + k] = R(0);
+      }
+    if (tj < kTJ)
+      for (int q = tid; q < (kTJ - tj) * ldv; q += blockDim.x) {
+        V[(size_t)tj * ldv + q] = R(0);
+        Vz[(size_t)tj * ldv + q] = R(0);
+      }
+  }
   double S[AF ? 1 : NPA][ZV];
 #pragma unroll
   for (int p = 0; p < (AF ? 1 : NPA); ++p) {
@@ -208,31 +266,48 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
   for (int q = 0; q < JH; ++q) acc[q] = R(0);
 
   const int rix = ld_uniform(tb.radius_index + ang);
-  const int i_lo = c * g.L, i_hi = min(g.P, i_lo + g.L);
-  // the chunk's PSF taps (z then y, per plane) staged in LDS: the blur phases
+  const int i_lo = pc.i_lo, i_hi = pc.i_hi;
+  // the piece's PSF taps (z then y, per plane) staged in LDS: the blur phases
   // read them as LDS broadcasts instead of waiting on scalar loads
   R *Wt = Vz + (size_t)kTJ * ldv;  // [i_hi - i_lo][2K]
+  const R lin_j = tid < tj ? tb.lin[j0 + tid] : R(0);
   if constexpr (H > 0) {
+    // all of a thread's tap loads are issued before the first LDS store
+    constexpr int TB = 8;
     const int nt = (i_hi - i_lo) * 2 * K;
-    for (int q = tid; q < nt; q += blockDim.x) {
-      const int l = q / (2 * K), r = q - l * (2 * K);
-      const size_t row = ((size_t)rix * g.P + i_lo + l) * K;
-      Wt[q] = r < K ? tb.taps_z[row + r] : tb.taps_r[row + r - K];
+    for (int q0 = tid; q0 < nt; q0 += TB * (int)blockDim.x) {
+      R w[TB];
+#pragma unroll
+      for (int u = 0; u < TB; ++u) {
+        const int q = min(q0 + u * (int)blockDim.x, nt - 1);
+        const int l = q / (2 * K), r = q - l * (2 * K);
+        const size_t row = ((size_t)rix * g.P + i_lo + l) * K;
+        w[u] = r < K ? tb.taps_z[row + r] : tb.taps_r[row + r - K];
+      }
+#pragma unroll
+      for (int u = 0; u < TB; ++u)
+        if (q0 + u * (int)blockDim.x < nt) Wt[q0 + u * blockDim.x] = w[u];
     }
   }
   // which planes of the chunk can touch the object (bit l: plane i_lo + l);
   // the skipped planes sample outside the object, where f and mu vanish
-  if (tid == 0) act_s = 0ull;
-  __syncthreads();
-  for (int q = tid; q < (i_hi - i_lo) * kTJ; q += blockDim.x) {
-    const int l = q / kTJ, jj = q - l * kTJ;
-    if (jj < tj) {
-      R ix, iy;
-      rot_coord<R>(cs.x, cs.y, tb.lin[j0 + jj], tb.lin[i_lo + l], g.half, ix, iy);
-      if (in_footprint<R>(ix, iy, g)) atomicOr(&act_s, 1ull << l);
+  if (pc.ranged) {
+    // every plane of the range
+    if (tid == 0) act_s = (i_hi - i_lo >= 64) ? ~0ull : ((1ull << (i_hi - i_lo)) - 1);
+  } else if (tb.act) {
+    if (tid == 0) act_s = ld_uniform(tb.act + ((size_t)ang * g.ntiles + jt) * g.C + c);
+  } else {
+    if (tid == 0) act_s = 0ull;
+    __syncthreads();
+    for (int q = tid; q < (i_hi - i_lo) * kTJ; q += blockDim.x) {
+      const int l = q / kTJ, jj = q - l * kTJ;
+      if (jj < tj) {
+        R ix, iy;
+        rot_coord<R>(cs.x, cs.y, tb.lin[j0 + jj], tb.lin[i_lo + l], g.half, ix, iy);
+        if (in_footprint<R>(ix, iy, g)) atomicOr(&act_s, 1ull << l);
+      }
     }
   }
-  const R lin_j = tid < tj ? tb.lin[j0 + tid] : R(0);
   // taps (and for TRB the contributor lists) of plane i into buffer b; run by tid < tj
   auto setup = [&](int i, int b) {
     R ix, iy;
@@ -517,12 +592,175 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, int
     b ^= 1;
   }
   if (cact) {
-    R *dst = partial + ((((size_t)c * n_call + al) * g.ntiles + jt) * JO) * Nz;
+    R *dst = partial + pc.slot * JO * Nz;
 #pragma unroll
     for (int q = 0; q < JH; ++q) {
       const int jo = half * JH + q;
       if (jo < JO) dst[(size_t)jo * Nz + zc] = acc[q];
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Balanced schedule.  Every (angle, tile) pair p = al * ntiles + jt visits a
+// contiguous range of planes [i_min, i_max] (the plan's activity table), n_p
+// planes.  The schedule splits each pair into c_p = ceil(n_p / Q) pieces of
+// nearly equal length, with the smallest Q <= kPieceMax for which all pieces
+// fit in the resident blocks at once (Q >= ceil(T / blocks), T = sum n_p), so
+// one launch runs as one round of blocks of at most Q planes each -- instead
+// of uniform chunks, whose central ones are full and edge ones nearly empty.
+// Block b takes piece b (pieces ordered by pair, then position): piece k of
+// pair p covers positions [k n_p / c_p, (k + 1) n_p / c_p) of the pair's
+// planes walked from i_max down, and writes partial slot b.
+//
+// sched layout (ints): [0] total pieces, [1] Q, then base[0..npairs]
+// (base[p] = pieces of the pairs before p).
+
+// One block computes the schedule; blockDim.x <= 1024, any number of pairs.
+template <typename R>
+__device__ __forceinline__ void schedule_body(const Geo<R> &g, const int32_t *__restrict__ angle_idx,
+                                              const int2 *__restrict__ ranges, int n, int nblocks,
+                                              int *__restrict__ sched) {
+  __shared__ int part[1024];
+  __shared__ int qs;
+  const int tid = threadIdx.x, bd = blockDim.x;
+  const int npairs = n * g.ntiles;
+  const int per = (npairs + bd - 1) / bd;
+  const int p0 = min(npairs, tid * per), p1 = min(npairs, p0 + per);
+  int *base = sched + 2;
+  auto plen = [&](int p) -> int {
+    const int al = p / g.ntiles, jt = p - al * g.ntiles;
+    const int ang = angle_idx[al];
+    if ((unsigned)ang >= (unsigned)g.A) return 0;
+    const int2 r = ranges[(size_t)ang * g.ntiles + jt];
+    return r.y >= r.x ? r.y - r.x + 1 : 0;
+  };
+  // block-wide sum (all threads get it)
+  auto block_sum = [&](int v) -> int {
+    part[tid] = v;
+    __syncthreads();
+    for (int d = bd >> 1; d > 0; d >>= 1) {
+      if (tid < d) part[tid] += part[tid + d];
+      __syncthreads();
+    }
+    const int r = part[0];
+    __syncthreads();
+    return r;
+  };
+  // exclusive block scan (Hillis-Steele)
+  auto block_scan = [&](int v) -> int {
+    part[tid] = v;
+    __syncthreads();
+    for (int d = 1; d < bd; d <<= 1) {
+      const int add = tid >= d ? part[tid - d] : 0;
+      __syncthreads();
+      part[tid] += add;
+      __syncthreads();
+    }
+    const int incl = part[tid];
+    __syncthreads();
+    return incl - v;
+  };
+  int t = 0;
+  for (int p = p0; p < p1; ++p) t += plen(p);
+  const int T = block_sum(t);
+  // smallest Q in [ceil(T / nblocks), kPieceMax] whose pieces fit nblocks
+  int q = max(1, min(kPieceMax, (T + nblocks - 1) / max(1, nblocks)));
+  for (;; ++q) {
+    int c = 0;
+    for (int p = p0; p < p1; ++p) c += (plen(p) + q - 1) / q;
+    if (block_sum(c) <= nblocks || q >= kPieceMax) break;
+  }
+  int cnt = 0;
+  for (int p = p0; p < p1; ++p) cnt += (plen(p) + q - 1) / q;
+  int b0 = block_scan(cnt);
+  for (int p = p0; p < p1; ++p) {
+    base[p] = b0;
+    b0 += (plen(p) + q - 1) / q;
+  }
+  if (tid == bd - 1) {
+    base[npairs] = b0;
+    sched[0] = b0;
+    sched[1] = q;
+  }
+}
+
+// Block b of the scheduled forward: piece b.
+template <typename R, int RZ, int QS, int H, bool AF>
+__device__ __forceinline__ void fwd_sched_body(const FwdJob<R> &J, const int2 *__restrict__ ranges,
+                                               const int *__restrict__ sched, int b,
+                                               unsigned char *smem) {
+  const Geo<R> &g = J.g;
+  const int npairs = J.n * g.ntiles;
+  const int *base = sched + 2;
+  if (b >= ld_uniform(sched)) return;
+  // the pair of piece b: the last p with base[p] <= b (a non-empty pair)
+  int lo = 0, hi = npairs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ld_uniform(base + mid) <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const int p = lo;
+  const int al = p / g.ntiles, jt = p - al * g.ntiles;
+  const int ang = ld_uniform(J.angle_idx + al);
+  const int2 r = ranges[(size_t)ang * g.ntiles + jt];
+  const int np = r.y - r.x + 1;
+  const int c = ld_uniform(base + p + 1) - ld_uniform(base + p);
+  const int k = b - ld_uniform(base + p);
+  const int a = (int)((long long)k * np / c), e = (int)((long long)(k + 1) * np / c);
+  FwdPiece pc;
+  pc.i_hi = r.y - a + 1;
+  pc.i_lo = r.y - (e - 1);
+  pc.c = -1;
+  pc.ranged = 1;
+  pc.slot = (size_t)b;
+  fwd_body<R, RZ, QS, H, false, AF>(J, al, jt, pc, smem);
+}
+
+// Reduce of the scheduled forward: the pieces of the <= 3 tiles a column
+// lies in, tiles ascending, pieces ascending (a fixed order).
+template <typename R>
+__device__ __forceinline__ void reduce_sched_body(const ReduceJob<R> &J, const int *__restrict__ sched,
+                                                  size_t idx) {
+  const Geo<R> &g = J.g;
+  const R *__restrict__ partial = J.partial;
+  const int n = J.n;
+  const int nout = g.jo1 - g.jo0;
+  const size_t count = (size_t)n * nout * g.Nz;
+  if (idx >= count) return;
+  const int z = idx % g.Nz;
+  const int jx = (idx / g.Nz) % nout;
+  const int al = idx / ((size_t)g.Nz * nout);
+  const int JO = kTJ + 2 * g.H;
+  const int rel = g.jo0 + jx - g.je0 + g.H;  // column within tile t: rel - t*kTJ
+  int t_hi = rel / kTJ;
+  if (t_hi > g.ntiles - 1) t_hi = g.ntiles - 1;
+  const int t_lo = (rel - JO) < 0 ? 0 : (rel - JO) / kTJ + 1;
+  const int *base = sched + 2;
+  const size_t plane = (size_t)JO * g.Nz;
+  R acc = R(0);
+  for (int t = t_lo; t <= t_hi; ++t) {
+    const int p = al * g.ntiles + t;
+    const int k0 = base[p], k1 = base[p + 1];
+    const R *src = partial + (size_t)(rel - t * kTJ) * g.Nz + z;
+    // 8 loads in flight; slots past k1 re-read a valid piece and are not added
+    constexpr int kRB = 8;
+    for (int k = k0; k < k1; k += kRB) {
+      R v[kRB];
+#pragma unroll
+      for (int u = 0; u < kRB; ++u) v[u] = src[(size_t)min(k + u, k1 - 1) * plane];
+#pragma unroll
+      for (int u = 0; u < kRB; ++u)
+        if (k + u < k1) acc += v[u];
+    }
+  }
+  if (J.proj) J.proj[idx] = acc;
+  if (J.gm) {
+    R p = acc;
+    if (J.s) p = p + J.s[idx];
+    if (J.pred) J.pred[idx] = p;
+    J.ratio[idx] = J.gm[idx] / (p + J.delta);
   }
 }
 

@@ -72,6 +72,10 @@ class SPECTSystemMatrix(SystemMatrix):
         chunk_planes: planes per kernel work unit (0: library default -- 16, and
             24 for the back projection's field kernel with the attenuation field;
             a positive value applies to every kernel).
+        sched_rounds: the forward projection without attenuation or with the
+            attenuation field splits its visited planes evenly over this many
+            rounds of the blocks the GPU holds at once (0: library default, 1);
+            -1 uses uniform chunks of ``chunk_planes`` planes instead.
         attenuation_field: keep the attenuation factor of every rotated-frame
             voxel of every angle resident in device memory (built once per
             attenuation map) instead of resampling mu in every projection.
@@ -86,6 +90,7 @@ class SPECTSystemMatrix(SystemMatrix):
     def __init__(self, obj2obj_transforms, proj2proj_transforms, object_meta, proj_meta,
                  object_initial_based_on_camera_path: bool = False,
                  sensitivity_dtype: torch.dtype = torch.float64, chunk_planes: int = 0,
+                 sched_rounds: int = 0,
                  attenuation_field: bool | None = None) -> None:
         super().__init__(object_meta, proj_meta, obj2obj_transforms, proj2proj_transforms)
         if object_initial_based_on_camera_path:
@@ -101,6 +106,7 @@ class SPECTSystemMatrix(SystemMatrix):
                              f"{(shape[1], shape[2])}")
         self._shape = shape
         self.chunk_planes = int(chunk_planes)
+        self.sched_rounds = int(sched_rounds)
         self.attenuation_field = attenuation_field
         self._plans: dict = {}
         self._ws: dict = {}
@@ -140,7 +146,8 @@ class SPECTSystemMatrix(SystemMatrix):
                     dy=float(self.object_meta.dy), dz=float(self.object_meta.dz),
                     angles_deg=_as_ptr(a32, ctypes.c_float),
                     angles_deg64=_as_ptr(a64, ctypes.c_double),
-                    psf_kr=0, psf_kz=0, n_radii=0, chunk_planes=self.chunk_planes)
+                    psf_kr=0, psf_kz=0, n_radii=0, chunk_planes=self.chunk_planes,
+                    sched_rounds=self.sched_rounds)
         if self._psf is not None:
             radii = np.asarray(self.proj_meta.radii, dtype=np.float64).reshape(-1)
             if radii.size == 1:

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "activity.hpp"
 #include "tiers.hpp"
 
 thread_local cpu_dim3 threadIdx;
@@ -80,9 +81,12 @@ struct Problem {
   std::vector<float2> trig;
   std::vector<float> lin, taps_r, taps_z, mu;
   std::vector<int32_t> ridx, sel, tier;
+  std::vector<uint64_t> act;
+  std::vector<int2> ranges;
   spect::Tables<float> tables() const {
     return {trig.data(), lin.data(), ridx.data(), H > 0 ? taps_r.data() : nullptr,
-            H > 0 ? taps_z.data() : nullptr, H > 0 ? tier.data() : nullptr};
+            H > 0 ? taps_z.data() : nullptr, H > 0 ? tier.data() : nullptr,
+            reinterpret_cast<const unsigned long long *>(act.data()), ranges.data()};
   }
 };
 
@@ -152,6 +156,23 @@ inline Problem load_problem(const std::string &dir) {
   const int halfway = p.P / 2;
   for (int k = 0; k < p.P; ++k)
     p.lin[k] = k < halfway ? -1.0f + step * float(k) : 1.0f - step * float(p.P - k - 1);
+  // planes each forward block visits (as spect_plan_create tabulates them)
+  {
+    std::vector<double> cv(p.A), sv(p.A), l64(p.P);
+    for (int a = 0; a < p.A; ++a) {
+      const double r64 = -(270.0 - (double)angles[a]) * M_PI / 180.0;
+      cv[a] = std::cos(r64);
+      sv[a] = std::sin(r64);
+    }
+    const double st64 = 2.0 / double(p.P - 1);
+    for (int k = 0; k < p.P; ++k)
+      l64[k] = k < halfway ? -1.0 + st64 * double(k) : 1.0 - st64 * double(p.P - k - 1);
+    std::vector<int> rng;
+    p.act = spect::forward_activity(cv.data(), sv.data(), p.A, l64.data(), p.P, g.je0, g.je1, p.L,
+                                    p.C, spect::kTJ, (double)g.lo, (double)g.hi, &rng);
+    p.ranges.resize(rng.size() / 2);
+    for (size_t k = 0; k < p.ranges.size(); ++k) p.ranges[k] = int2{rng[2 * k], rng[2 * k + 1]};
+  }
   return p;
 }
 

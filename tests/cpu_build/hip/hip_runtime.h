@@ -41,6 +41,10 @@ inline float __expf(float v) { return std::exp(v); }
 struct float2 {
   float x, y;
 };
+struct int2 {
+  int x, y;
+};
+inline int2 make_int2(int a, int b) { return {a, b}; }
 struct double2 {
   double x, y;
 };

@@ -158,8 +158,11 @@ def _mix_likelihoods(cuda):
 
 def test_uneven_mix_batched_vs_one_at_a_time_and_oracle(cuda):
     """Every problem of the mix: the batched hipGraph run (one iteration per
-    replay, 4 x 8) equals its one-at-a-time OSEM bit for bit; the 64^3 x 60
-    problem also matches the CPU oracle after every iteration (rel-L2 1e-5)."""
+    replay, 4 x 8) matches its one-at-a-time OSEM to 1e-6 rel-L2 (the same
+    per-plane arithmetic; the one-at-a-time forward adds its partial ray sums
+    in the balanced schedule's pieces, the batch in uniform chunks, so the
+    float32 sums are rounded in a different order); the 64^3 x 60 problem also
+    matches the CPU oracle after every iteration (rel-L2 1e-5)."""
     from oracle import spect_oracle as so
     from pytomography_amd.algorithms import OSEM
     from pytomography_amd.batch import BatchedOSEM
@@ -178,10 +181,10 @@ def test_uneven_mix_batched_vs_one_at_a_time_and_oracle(cuda):
         iterates.append([o.clone() for o in batch.objects])
     torch.cuda.synchronize()
     for (n, na, _), a, b in zip(MIX, iterates[-1], ref):
-        same = torch.equal(a, b)
-        print(f"{n}^3 x {na}: batched == one-at-a-time: {same} "
-              f"(rel-L2 {rel_l2(a, b):.1e})")
-        assert same
+        e = rel_l2(a, b)
+        print(f"{n}^3 x {na}: batched vs one-at-a-time rel-L2 {e:.1e}, "
+              f"max abs {float((a - b).abs().max()):.1e}")
+        assert e <= 1e-6
     pa, lk = probs[0]
     g32 = oracle_geometry(pa, torch.float32)
     data = lk.projections.cpu()

@@ -100,12 +100,14 @@ def _exec(exe, d, extra=()):
     return dict(zip(words[0::2], map(int, words[1::2])))
 
 
-def _run(exe, tmp_path, pa, sel, chunk_planes=16, att_mode=0, trb=False, obj=None):
+def _run(exe, tmp_path, pa, sel, chunk_planes=16, att_mode=0, trb=False, obj=None, sched_blocks=None):
     """Forward of pa["f"] (or ``obj``) over the angles ``sel`` by the CPU
-    build; with ``trb`` the B^T mode of the same body."""
+    build; with ``trb`` the B^T mode of the same body, with ``sched_blocks``
+    the balanced schedule over that many blocks."""
     _write_problem(tmp_path, pa, sel, chunk_planes, att_mode)
     np.ascontiguousarray(pa["f"] if obj is None else obj, dtype=np.float32).tofile(tmp_path / "f.bin")
-    stats = _exec(exe, tmp_path, ["trb"] if trb else [])
+    extra = ["trb"] if trb else (["sched", str(sched_blocks)] if sched_blocks else [])
+    stats = _exec(exe, tmp_path, extra)
     proj = np.fromfile(tmp_path / "proj.bin", dtype=np.float32).reshape(len(sel), pa["n"], pa["nz"])
     return torch.from_numpy(proj), stats
 
@@ -136,6 +138,26 @@ def test_forward_body_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel
     assert stats["nan_partial"] == 0
     assert bool(torch.isfinite(proj).all()) and bool((proj >= 0).all())
     assert max(errs) <= 1e-5  # the per-projection bar of the GPU parity tests
+
+
+@pytest.mark.parametrize("blocks", [1, 5, 37])
+@pytest.mark.parametrize("name,spec,sel,att_mode", [c for c in CASES if c[3] != 2],
+                         ids=[c[0] for c in CASES if c[3] != 2])
+def test_balanced_schedule_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel, att_mode, blocks):
+    """The balanced schedule (one-block schedule kernel, blocks walking equal
+    runs of the concatenated plane ranges, the piece reduce) on 1, 5 and 37
+    blocks: pieces of one (angle, tile) pair split across blocks and blocks
+    spanning several pairs.  Every slot the reduce reads is written (the
+    partials start as NaN) and the projections match the oracle."""
+    spec = dict(spec)
+    pa = problem_arrays(spec.pop("n"), spec.pop("n_angles"), seed=5, **spec)
+    proj, stats = _run(cpu_build["fwd_cpu"], tmp_path, pa, sel, att_mode=att_mode, sched_blocks=blocks)
+    ref = so.forward(oracle_geometry(pa), torch.from_numpy(pa["f"]), torch.tensor(sel))
+    errs = [rel_l2(proj[k], ref[k]) for k in range(len(sel))]
+    print(f"{name}, {blocks} blocks: {stats['slots']} pieces, max rel-L2 {max(errs):.1e}")
+    assert stats["nan_partial"] == 0
+    assert bool(torch.isfinite(proj).all()) and bool((proj >= 0).all())
+    assert max(errs) <= 1e-5
 
 
 @pytest.mark.skipif(os.environ.get("SPECT_CPU_BUILD_FULL") != "1",
