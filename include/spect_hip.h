@@ -72,7 +72,7 @@ typedef struct spect_plan spect_plan;
  * and the angle handling of SPECTSystemMatrix.forward/backward
  * (projectors/SPECT/spect_system_matrix.py:145-154,188-198). */
 typedef struct spect_geom {
-  int32_t nx, ny, nz;      /* object shape; nx == ny (square transaxial grid) */
+  int32_t nx, ny, nz;      /* object shape; nx == ny <= 16384, nz <= 256 */
   int32_t n_angles;        /* number of projections */
   double dx, dy, dz;       /* voxel size in cm (dx scales the attenuation integral) */
   const float *angles_deg; /* host [n_angles]: detector angle beta (float32, as proj_meta.angles) */

@@ -20,6 +20,11 @@
 //   gather, written as a gather over the <= 16 rotated-frame candidates of
 //   each voxel (race-free and deterministic); MODE 2 adds the mu gradient.
 // The block bodies (field_body, gather_body) are in backward_body.hpp.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "backward_body.hpp"
 
 namespace spect {
@@ -42,6 +47,12 @@ __global__ __launch_bounds__(1024) void field_batch_kernel(const FieldJob<R> *__
   const int local = b - ld_uniform(starts + j);
   const int c = local % J.g.C, r = local / J.g.C;
   field_body<R, RZ, QS, H, false, AF>(J, r / J.g.ntiles, r % J.g.ntiles, c, smem);
+}
+
+template <typename R, int RZ, int QS, int H, bool AF>
+__global__ __launch_bounds__(1024) void field_sched_kernel(FieldJob<R> J, int nblocks) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  field_sched_body<R, RZ, QS, H, AF>(J, nblocks, blockIdx.x, smem);
 }
 
 // In place: Dsum[a][c] <- sum_{c' <= c} Dsum[a][c'] (prefix over chunks, FP64).
@@ -106,7 +117,90 @@ hipError_t launch_field_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, c
   return hipGetLastError();
 }
 
+// Dynamic LDS of the field kernels: U ([kTJ][stride]) and the taps of up
+// to max(L, kPieceMax) planes.
+template <typename R, int RZ, int H> size_t field_smem_bytes(int Nz, int L) {
+  const int runs = (Nz + RZ - 1) / RZ;
+  const int planes = L > kPieceMax ? L : kPieceMax;
+  return ((size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)planes * 2 * (2 * H + 1)) * sizeof(R);
+}
+
+template <typename R, int H, bool AF>
+int field_resident_blocks(int Nz, int L) {
+  constexpr int RZ = BpTile<R>::RZ, QS = BpTile<R>::QS;
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find({dev, Nz, L});
+  if (it != cache.end()) return it->second;
+  const int bt = tile_block_threads(Nz, RZ, QS);
+  const size_t smem = field_smem_bytes<R, RZ, H>(Nz, L);
+  auto kern = field_sched_kernel<R, RZ, QS, H, AF>;
+  if (smem > 64 * 1024 &&
+      hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)smem) != hipSuccess)
+    return 0;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, bt, smem) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const int nb = per_cu > 0 && cus > 0 ? per_cu * cus : 0;
+  cache[{dev, Nz, L}] = nb;
+  return nb;
+}
+
+template <typename R, int H>
+hipError_t launch_field_sched_h(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *attf,
+                                const int32_t *idx, int n, R *W, int nb, hipStream_t st) {
+  constexpr int RZ = BpTile<R>::RZ, QS = BpTile<R>::QS;
+  const int bt = tile_block_threads(g.Nz, RZ, QS);
+  if (bt > 1024) return hipErrorInvalidConfiguration;  // __launch_bounds__(1024)
+  if (nb <= 0) return hipErrorInvalidConfiguration;
+  const size_t smem = field_smem_bytes<R, RZ, H>(g.Nz, g.L);
+  auto kern = attf ? field_sched_kernel<R, RZ, QS, H, true> : field_sched_kernel<R, RZ, QS, H, false>;
+  if (smem > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+  }
+  const int grid = sched_grid_blocks(n, g.ntiles, g.P, nb);
+  kern<<<grid, bt, smem, st>>>(
+      FieldJob<R>{g, tb, proj, nullptr, nullptr, 0, attf, idx, n, W, nullptr, nullptr, nullptr}, nb);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+template <typename R>
+int field_sched_blocks(const Geo<R> &g) {
+#define SPECT_FSB_CASE(HH)                                                                   \
+  case HH:                                                                                    \
+    return std::max(field_resident_blocks<R, HH, false>(g.Nz, g.L),                         \
+                    field_resident_blocks<R, HH, true>(g.Nz, g.L));
+  switch (g.H) {
+    SPECT_FSB_CASE(0) SPECT_FSB_CASE(2) SPECT_FSB_CASE(4) SPECT_FSB_CASE(6) SPECT_FSB_CASE(8)
+    SPECT_FSB_CASE(10) SPECT_FSB_CASE(12) SPECT_FSB_CASE(14) SPECT_FSB_CASE(16)
+    SPECT_FSB_CASE(18) SPECT_FSB_CASE(20)
+    default: return 0;
+  }
+#undef SPECT_FSB_CASE
+}
+
+template <typename R>
+hipError_t launch_bp_field_sched(const Geo<R> &g, const Tables<R> &tb, const R *proj,
+                                 const R *attf, const int32_t *idx, int n, R *W, int blocks,
+                                 hipStream_t st) {
+#define SPECT_FSL_CASE(HH) \
+  case HH: return launch_field_sched_h<R, HH>(g, tb, proj, attf, idx, n, W, blocks, st);
+  switch (g.H) {
+    SPECT_FSL_CASE(0) SPECT_FSL_CASE(2) SPECT_FSL_CASE(4) SPECT_FSL_CASE(6) SPECT_FSL_CASE(8)
+    SPECT_FSL_CASE(10) SPECT_FSL_CASE(12) SPECT_FSL_CASE(14) SPECT_FSL_CASE(16)
+    SPECT_FSL_CASE(18) SPECT_FSL_CASE(20)
+    default: return hipErrorInvalidValue;
+  }
+#undef SPECT_FSL_CASE
+}
 
 template <typename R>
 hipError_t launch_bp_field(const Geo<R> &g, const Tables<R> &tb, const R *proj, const R *mu,
@@ -200,6 +294,10 @@ hipError_t launch_gather_batch(const GatherJob<R> *jobs, const int *starts, int 
                                          const int32_t *, int, R *, const R *, double *,        \
                                          double *, hipStream_t);                                \
   template hipError_t launch_chunk_prefix<R>(const Geo<R> &, double *, int, hipStream_t);      \
+  template int field_sched_blocks<R>(const Geo<R> &);                                          \
+  template hipError_t launch_bp_field_sched<R>(const Geo<R> &, const Tables<R> &, const R *,    \
+                                               const R *, const int32_t *, int, R *, int,       \
+                                               hipStream_t);                                   \
   template hipError_t launch_field_batch<R>(const FieldJob<R> *, const int *, int, int, int,    \
                                             int, int, bool, hipStream_t);                      \
   template hipError_t launch_gather_batch<R>(const GatherJob<R> *, const int *, int, int, int,  \

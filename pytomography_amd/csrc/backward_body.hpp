@@ -18,9 +18,11 @@ namespace spect {
 // sum_{i>k} D(i) + D(k)/2 - sum_i D(i)  (M = mu_rot*dx).  The block writes the
 // chunk-local part Wm(k) = sum_{i>k, i in chunk} D(i) + D(k)/2 (FP64) and the
 // chunk total Dsum (FP64); the gather subtracts the prefix of chunk totals.
+// Planes [i_lo, i_hi) of (angle al, tile jt); c is the chunk whose FP64
+// suffix-table rows start the attenuation march (suffix-table mode only).
 template <typename R, int RZ, int QS, int H, bool GRAD, bool AF>
-__device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt, int c,
-                                           unsigned char *smem) {
+__device__ __forceinline__ void field_body_range(const FieldJob<R> &J, int al, int jt, int c,
+                                                 int i_lo, int i_hi, unsigned char *smem) {
   const Geo<R> &g = J.g;
   const Tables<R> &tb = J.tb;
   const R *__restrict__ proj = J.proj;
@@ -82,7 +84,6 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
 #pragma unroll
   for (int m = 0; m < (GRAD ? RZ : 1); ++m) SD[m] = 0.0;
   const int rix = ld_uniform(tb.radius_index + ang);
-  const int i_lo = c * g.L, i_hi = min(g.P, i_lo + g.L);
   // the chunk's PSF taps (z then y, per plane) staged in LDS: the blur phases
   // read them as LDS broadcasts instead of waiting on scalar loads
   R *Wt = U + (size_t)kTJ * ldu;  // [i_hi - i_lo][2K]
@@ -225,6 +226,31 @@ __device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt,
           Dsum[(((size_t)al * g.C + c) * n_jext + (j0 - g.je0 + jl)) * Nz + z0 + m] = SD[m];
     }
   }
+}
+
+// Chunk c of the uniform schedule: planes [c L, c L + L).
+template <typename R, int RZ, int QS, int H, bool GRAD, bool AF>
+__device__ __forceinline__ void field_body(const FieldJob<R> &J, int al, int jt, int c,
+                                           unsigned char *smem) {
+  const int i_lo = c * J.g.L;
+  field_body_range<R, RZ, QS, H, GRAD, AF>(J, al, jt, c, i_lo, min(J.g.P, i_lo + J.g.L), smem);
+}
+
+// Block b of the balanced field schedule (sched_locate in forward_body.hpp:
+// the forward's pairs and plane ranges, split for this kernel's resident
+// blocks): piece b, planes walked from the top of its range.
+template <typename R, int RZ, int QS, int H, bool AF>
+__device__ __forceinline__ void field_sched_body(const FieldJob<R> &J, int nblocks, int b,
+                                                 unsigned char *smem) {
+  const Geo<R> &g = J.g;
+  const SchedLoc s = sched_locate<R>(g, J.angle_idx, J.tb.ranges, J.n, nblocks, b, nullptr);
+  if (s.p < 0) return;
+  const int al = s.p / g.ntiles, jt = s.p - al * g.ntiles;
+  const int ang = ld_uniform(J.angle_idx + al);
+  const int2 r = J.tb.ranges[(size_t)ang * g.ntiles + jt];
+  int i_lo, i_hi;
+  sched_piece_planes(r, s.k, s.c, i_lo, i_hi);
+  field_body_range<R, RZ, QS, H, false, AF>(J, al, jt, -1, i_lo, i_hi, smem);
 }
 
 // Output stage of a back projection for one voxel value b.

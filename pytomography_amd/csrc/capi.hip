@@ -311,8 +311,14 @@ int backward_impl(const spect_plan *p, int mode, int out_mode, const R *proj, co
     gf.L = p->L_field;
     gf.C = (p->P + gf.L - 1) / gf.L;
   }
-  HIP_TRY(launch_bp_field<R>(gf, tables<R>(p), proj, mu, att, global, attf, idx, n, W, nullptr,
-                             nullptr, nullptr, st));
+  const int fb = (attf || !mu) && p->rounds >= 0 ? p->rounds * field_sched_blocks<R>(gf) : 0;
+  if (fb > 0) {
+    // the balanced schedule (as the forward's, for this kernel's blocks)
+    HIP_TRY(launch_bp_field_sched<R>(gf, tables<R>(p), proj, attf, idx, n, W, fb, st));
+  } else {
+    HIP_TRY(launch_bp_field<R>(gf, tables<R>(p), proj, mu, att, global, attf, idx, n, W, nullptr,
+                               nullptr, nullptr, st));
+  }
   HIP_TRY(launch_bp_gather<R>(g, tables<R>(p), W, idx, n, mode, out_mode, out, f, nrm, R(delta),
                               R(floor_value), nullptr, nullptr, nullptr, st));
   return SPECT_OK;
@@ -616,6 +622,7 @@ int spect_plan_create(const spect_geom *g, spect_plan **out) {
   if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0) return fail(SPECT_E_ARG, "object shape must be positive");
   if (g->nx != g->ny) return fail(SPECT_E_ARG, "nx must equal ny (square transaxial grid)");
   if (g->nz > 256) return fail(SPECT_E_ARG, "nz > 256 is not supported by the tile kernels");
+  if (g->nx > 16384) return fail(SPECT_E_ARG, "nx > 16384 is not supported");
   if (g->chunk_planes < 0 || g->chunk_planes > 64)
     return fail(SPECT_E_ARG, "chunk_planes must be 0 (default) or 1..64");
   if (g->sched_rounds < -1 || g->sched_rounds > 16)

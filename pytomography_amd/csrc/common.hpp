@@ -344,4 +344,189 @@ __device__ __forceinline__ bool in_footprint(R ix, R iy, const Geo<R> &g) {
   return ix > g.lo && ix < g.hi && iy > g.lo && iy < g.hi;
 }
 
+// ---------------------------------------------------------------------------
+// Balanced schedule (used by the forward and the back-projection field):
+// pair p = al * ntiles + jt visits the n_p planes of its range [i_min, i_max]
+// in the plan's activity table; it is split into c_p = ceil(n_p / Q) pieces,
+// Q the smallest plane count <= kPieceMax for which all pieces fit the
+// resident blocks at once.
+
+// Wave-level sums over the 64 lanes by DPP moves (VALU operations; no LDS
+// round trips).  Called from control flow that every lane of the wave
+// reaches.  dpp_move<C>: the value of the lane that DPP control C selects;
+// 0 where that lane lies outside the row of 16.
+template <int C>
+__device__ __forceinline__ int dpp_move(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, C, 0xF, 0xF, false);
+}
+
+// Sum over the 64 lanes of a wave (every lane gets it): two butterflies
+// within quads (quad_perm [1,0,3,2], [2,3,0,1]) and row rotations by 4 and 8
+// leave every lane with its row's total; then the four row totals.
+__device__ __forceinline__ int wave_sum(int v) {
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x124>(v);
+  v += dpp_move<0x128>(v);
+  return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) +
+         __builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48);
+}
+
+// Inclusive prefix sum over the lanes of a wave: Hillis-Steele within each row
+// (row_shr by 1, 2, 4, 8), then the totals of the rows before.
+__device__ __forceinline__ int wave_scan(int v) {
+  v += dpp_move<0x111>(v);
+  v += dpp_move<0x112>(v);
+  v += dpp_move<0x114>(v);
+  v += dpp_move<0x118>(v);
+  const int lane = threadIdx.x & 63;
+  const int r0 = __builtin_amdgcn_readlane(v, 15), r1 = __builtin_amdgcn_readlane(v, 31),
+            r2 = __builtin_amdgcn_readlane(v, 47);
+  return v + (lane >= 16 ? r0 : 0) + (lane >= 32 ? r1 : 0) + (lane >= 48 ? r2 : 0);
+}
+
+// Piece of one block: pair p (< 0: the block has none), its index k within
+// the pair and the pair's piece count c.
+struct SchedLoc {
+  int p, k, c;
+};
+
+// The schedule for nblocks resident blocks, evaluated by the whole block
+// (blockDim.x a multiple of 64, at most 1024; every thread calls), located at
+// piece b.  With `out`, block b = 0 writes the schedule there.
+template <typename R>
+__device__ __forceinline__ SchedLoc sched_locate(const Geo<R> &g, const int32_t *__restrict__ angle_idx,
+                                                 const int2 *__restrict__ ranges, int n, int nblocks,
+                                                 int b, int *__restrict__ out) {
+  __shared__ int red[16 * 8];  // per-wave partial sums
+  __shared__ int loc[3];
+  const int tid = threadIdx.x, bd = blockDim.x;
+  const int w = tid >> 6, lane = tid & 63, nw = bd >> 6;
+  const int npairs = n * g.ntiles;
+  // thread tid owns the contiguous pairs [p0, p1)
+  const int per = (npairs + bd - 1) / bd;
+  const int p0 = min(npairs, tid * per), p1 = min(npairs, p0 + per);
+  auto plen = [&](int p) -> int {
+    const int al = p / g.ntiles, jt = p - al * g.ntiles;
+    const int ang = angle_idx[al];
+    if ((unsigned)ang >= (unsigned)g.A) return 0;
+    const int2 r = ranges[(size_t)ang * g.ntiles + jt];
+    return r.y >= r.x ? r.y - r.x + 1 : 0;
+  };
+  // the first kKeep lengths stay in registers; threads (whole waves, when
+  // the pairs are fewer than the threads) without pairs skip every pass
+  constexpr int kKeep = 4;
+  int keep[kKeep];
+#pragma unroll
+  for (int i = 0; i < kKeep; ++i) keep[i] = p0 + i < p1 ? plen(p0 + i) : 0;
+  auto each = [&](auto fn) {
+#pragma unroll
+    for (int i = 0; i < kKeep; ++i)
+      if (p0 + i < p1) fn(p0 + i, keep[i]);
+    for (int p = p0 + kKeep; p < p1; ++p) fn(p, plen(p));
+  };
+  // ceil(len / q) = floor((a + 0.5) / q), a = len + q - 1, in float: the
+  // quotient lies at least 0.5 / q from an integer and the two roundings
+  // (1 / q, the product) move it by under (a / q + 1) 2^-23 -- exact for
+  // a < 2^20 (len <= P).  No integer division on the hot start of a block.
+  auto pieces = [](int len, int q, float inv_q) {
+    return (int)(((float)(len + q - 1) + 0.5f) * inv_q);
+  };
+
+  // T = sum of n_p
+  int t = 0;
+  each([&](int, int len) { t += len; });
+  t = wave_sum(t);
+  if (lane == 0) red[w] = t;
+  if (tid == 0) loc[0] = -1;
+  __syncthreads();
+  int T = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if (i < nw) T += red[i];
+  __syncthreads();
+  // smallest Q in [ceil(T / nblocks), kPieceMax] whose pieces fit nblocks,
+  // eight candidates per pass (nearly always one pass)
+  int q = kPieceMax;
+  for (int q0 = max(1, min(kPieceMax, (T + nblocks - 1) / nblocks)); q0 <= kPieceMax; q0 += 8) {
+    int s[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] = 0;
+    if (p0 < p1) {
+      float inv[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) inv[k] = 1.0f / (float)min(q0 + k, kPieceMax);
+      each([&](int, int len) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] += pieces(len, min(q0 + k, kPieceMax), inv[k]);
+      });
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      s[k] = wave_sum(s[k]);
+      if (lane == 0) red[w * 8 + k] = s[k];
+    }
+    __syncthreads();
+    int found = 0;  // 1 + the first fitting candidate
+#pragma unroll
+    for (int k = 7; k >= 0; --k) {
+      int S = 0;
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i < nw) S += red[i * 8 + k];
+      if (S <= nblocks) found = k + 1;
+    }
+    __syncthreads();
+    if (found) {
+      q = min(q0 + found - 1, kPieceMax);
+      break;
+    }
+  }
+  // pieces before each thread's pairs, then the owner of piece b
+  const float inv_q = 1.0f / (float)q;
+  int cnt = 0;
+  each([&](int, int len) { cnt += pieces(len, q, inv_q); });
+  const int incl = wave_scan(cnt);
+  if (lane == 63) red[w] = incl;
+  __syncthreads();
+  int pre = 0, total = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int v = i < nw ? red[i] : 0;
+    if (i < w) pre += v;
+    total += v;
+  }
+  int b0 = pre + incl - cnt;
+  const bool write = out != nullptr && b == 0;
+  if (write || (b >= b0 && b < b0 + cnt)) {
+    each([&](int p, int len) {
+      if (p >= p1) return;
+      const int c = pieces(len, q, inv_q);
+      if (write) out[2 + p] = b0;
+      if (b >= b0 && b < b0 + c) {
+        loc[0] = p;
+        loc[1] = b - b0;
+        loc[2] = c;
+      }
+      b0 += c;
+    });
+  }
+  if (write && tid == 0) {
+    out[0] = total;
+    out[1] = q;
+    out[2 + npairs] = total;
+  }
+  __syncthreads();
+  return {__builtin_amdgcn_readfirstlane(loc[0]), __builtin_amdgcn_readfirstlane(loc[1]),
+          __builtin_amdgcn_readfirstlane(loc[2])};
+}
+
+// Planes of piece k of c over the range r (walked from r.y down).
+__device__ __forceinline__ void sched_piece_planes(int2 r, int k, int c, int &i_lo, int &i_hi) {
+  const int np = r.y - r.x + 1;
+  const int a = (int)((long long)k * np / c), e = (int)((long long)(k + 1) * np / c);
+  i_hi = r.y - a + 1;
+  i_lo = r.y - (e - 1);
+}
+
 }  // namespace spect

@@ -65,19 +65,12 @@ __global__ __launch_bounds__(512) void fwd_kernel(FwdJob<R> J) {
                                   smem);
 }
 
-// The balanced schedule of a call (one block) and the forward blocks that
-// walk it (schedule_body / fwd_sched_body).
-template <typename R>
-__global__ __launch_bounds__(1024) void schedule_kernel(Geo<R> g, const int32_t *__restrict__ idx,
-                                                        const int2 *__restrict__ ranges, int n,
-                                                        int nblocks, int *__restrict__ sched) {
-  schedule_body<R>(g, idx, ranges, n, nblocks, sched);
-}
-
+// The forward blocks of the balanced schedule (fwd_sched_body), each locating
+// its piece for nblocks resident blocks; block 0 writes the schedule to sched.
 template <typename R, int RZ, int QS, int H, bool AF>
-__global__ __launch_bounds__(512) void fwd_sched_kernel(FwdJob<R> J, const int *__restrict__ sched) {
+__global__ __launch_bounds__(512) void fwd_sched_kernel(FwdJob<R> J, int *__restrict__ sched, int nblocks) {
   extern __shared__ __align__(16) unsigned char smem[];
-  fwd_sched_body<R, RZ, QS, H, AF>(J, J.tb.ranges, sched, blockIdx.x, smem);
+  fwd_sched_body<R, RZ, QS, H, AF>(J, J.tb.ranges, sched, nblocks, blockIdx.x, smem);
 }
 
 template <typename R>
@@ -205,11 +198,8 @@ hipError_t launch_fwd_sched_h(const Geo<R> &g, const Tables<R> &tb, const R *f, 
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
   }
-  const int npairs = n * g.ntiles;
-  const int sbt = npairs >= 1024 ? 1024 : ((npairs + 63) / 64) * 64;
-  schedule_kernel<R><<<1, sbt, 0, st>>>(g, idx, tb.ranges, n, nb, sched);
   FwdJob<R> J{g, tb, f, nullptr, nullptr, 0, attf, idx, n, partial};
-  kern<<<grid, bt, smem, st>>>(J, sched);
+  kern<<<grid, bt, smem, st>>>(J, sched, nb);
   return hipGetLastError();
 }
 

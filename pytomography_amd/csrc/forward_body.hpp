@@ -613,105 +613,26 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
 // pair p covers positions [k n_p / c_p, (k + 1) n_p / c_p) of the pair's
 // planes walked from i_max down, and writes partial slot b.
 //
+// Every block of a scheduled launch computes the schedule itself (a few
+// hundred pairs: two loads per pair, wave reductions and four barriers), so
+// no separate launch precedes it; block 0 also writes it out for the reduce.
 // sched layout (ints): [0] total pieces, [1] Q, then base[0..npairs]
 // (base[p] = pieces of the pairs before p).
 
-// One block computes the schedule; blockDim.x <= 1024, any number of pairs.
-template <typename R>
-__device__ __forceinline__ void schedule_body(const Geo<R> &g, const int32_t *__restrict__ angle_idx,
-                                              const int2 *__restrict__ ranges, int n, int nblocks,
-                                              int *__restrict__ sched) {
-  __shared__ int part[1024];
-  __shared__ int qs;
-  const int tid = threadIdx.x, bd = blockDim.x;
-  const int npairs = n * g.ntiles;
-  const int per = (npairs + bd - 1) / bd;
-  const int p0 = min(npairs, tid * per), p1 = min(npairs, p0 + per);
-  int *base = sched + 2;
-  auto plen = [&](int p) -> int {
-    const int al = p / g.ntiles, jt = p - al * g.ntiles;
-    const int ang = angle_idx[al];
-    if ((unsigned)ang >= (unsigned)g.A) return 0;
-    const int2 r = ranges[(size_t)ang * g.ntiles + jt];
-    return r.y >= r.x ? r.y - r.x + 1 : 0;
-  };
-  // block-wide sum (all threads get it)
-  auto block_sum = [&](int v) -> int {
-    part[tid] = v;
-    __syncthreads();
-    for (int d = bd >> 1; d > 0; d >>= 1) {
-      if (tid < d) part[tid] += part[tid + d];
-      __syncthreads();
-    }
-    const int r = part[0];
-    __syncthreads();
-    return r;
-  };
-  // exclusive block scan (Hillis-Steele)
-  auto block_scan = [&](int v) -> int {
-    part[tid] = v;
-    __syncthreads();
-    for (int d = 1; d < bd; d <<= 1) {
-      const int add = tid >= d ? part[tid - d] : 0;
-      __syncthreads();
-      part[tid] += add;
-      __syncthreads();
-    }
-    const int incl = part[tid];
-    __syncthreads();
-    return incl - v;
-  };
-  int t = 0;
-  for (int p = p0; p < p1; ++p) t += plen(p);
-  const int T = block_sum(t);
-  // smallest Q in [ceil(T / nblocks), kPieceMax] whose pieces fit nblocks
-  int q = max(1, min(kPieceMax, (T + nblocks - 1) / max(1, nblocks)));
-  for (;; ++q) {
-    int c = 0;
-    for (int p = p0; p < p1; ++p) c += (plen(p) + q - 1) / q;
-    if (block_sum(c) <= nblocks || q >= kPieceMax) break;
-  }
-  int cnt = 0;
-  for (int p = p0; p < p1; ++p) cnt += (plen(p) + q - 1) / q;
-  int b0 = block_scan(cnt);
-  for (int p = p0; p < p1; ++p) {
-    base[p] = b0;
-    b0 += (plen(p) + q - 1) / q;
-  }
-  if (tid == bd - 1) {
-    base[npairs] = b0;
-    sched[0] = b0;
-    sched[1] = q;
-  }
-}
-
-// Block b of the scheduled forward: piece b.
+// Block b of the scheduled forward: piece b, written to partial slot b; block
+// 0 writes the schedule to `sched` for the reduce.
 template <typename R, int RZ, int QS, int H, bool AF>
 __device__ __forceinline__ void fwd_sched_body(const FwdJob<R> &J, const int2 *__restrict__ ranges,
-                                               const int *__restrict__ sched, int b,
+                                               int *__restrict__ sched, int nblocks, int b,
                                                unsigned char *smem) {
   const Geo<R> &g = J.g;
-  const int npairs = J.n * g.ntiles;
-  const int *base = sched + 2;
-  if (b >= ld_uniform(sched)) return;
-  // the pair of piece b: the last p with base[p] <= b (a non-empty pair)
-  int lo = 0, hi = npairs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ld_uniform(base + mid) <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  const int p = lo;
-  const int al = p / g.ntiles, jt = p - al * g.ntiles;
+  const SchedLoc s = sched_locate<R>(g, J.angle_idx, ranges, J.n, nblocks, b, sched);
+  if (s.p < 0) return;
+  const int al = s.p / g.ntiles, jt = s.p - al * g.ntiles;
   const int ang = ld_uniform(J.angle_idx + al);
   const int2 r = ranges[(size_t)ang * g.ntiles + jt];
-  const int np = r.y - r.x + 1;
-  const int c = ld_uniform(base + p + 1) - ld_uniform(base + p);
-  const int k = b - ld_uniform(base + p);
-  const int a = (int)((long long)k * np / c), e = (int)((long long)(k + 1) * np / c);
   FwdPiece pc;
-  pc.i_hi = r.y - a + 1;
-  pc.i_lo = r.y - (e - 1);
+  sched_piece_planes(r, s.k, s.c, pc.i_lo, pc.i_hi);
   pc.c = -1;
   pc.ranged = 1;
   pc.slot = (size_t)b;

@@ -126,16 +126,24 @@ hipError_t launch_forward(const Geo<R> &g, const Tables<R> &tb, const R *f, cons
                           const int32_t *angle_idx, int n, R *partial, bool trb, hipStream_t st);
 
 // Balanced-schedule forward (no attenuation, or the attenuation field; no
-// B^T): a one-block kernel writes the schedule into sched (ints:
-// 2 + n * ntiles + 1), then sched_grid_blocks(n, ntiles, P,
-// forward_sched_blocks(g)) blocks write partial[slot][kTJ + 2H][Nz], one slot
-// each, with blocks = a multiple of forward_sched_blocks(g), the blocks
-// resident at once (0: the device cannot run the kernel).
+// B^T): sched_grid_blocks(n, ntiles, P, blocks) blocks, each locating its
+// piece of the schedule for `blocks` resident blocks (a multiple of
+// forward_sched_blocks(g), the blocks resident at once; 0: the device cannot
+// run the kernel), write partial[slot][kTJ + 2H][Nz], one slot each; block 0
+// writes the schedule into sched (ints: 2 + n * ntiles + 1) for the reduce.
 template <typename R> int forward_sched_blocks(const Geo<R> &g);
 template <typename R>
 hipError_t launch_forward_sched(const Geo<R> &g, const Tables<R> &tb, const R *f, const R *attf,
                                 const int32_t *angle_idx, int n, R *partial, int *sched,
                                 int blocks, hipStream_t st);
+// Back-projection field over the balanced schedule (attenuation field or no
+// attenuation): sched_grid_blocks(n, ntiles, P, blocks) blocks, one piece each;
+// field_sched_blocks(g) = its blocks resident at once (0: cannot run).
+template <typename R> int field_sched_blocks(const Geo<R> &g);
+template <typename R>
+hipError_t launch_bp_field_sched(const Geo<R> &g, const Tables<R> &tb, const R *proj,
+                                 const R *attf, const int32_t *angle_idx, int n, R *W, int blocks,
+                                 hipStream_t st);
 template <typename R>
 hipError_t launch_reduce_sched(const Geo<R> &g, const R *partial, int n, const int *sched, R *proj,
                                const R *g_meas, const R *s, R *pred, R *ratio, R delta,

@@ -18,6 +18,9 @@
 //                      call's angles, the field with the mu-gradient parts
 //                      (GRAD), the prefix of chunk totals, gather mode 2 --
 //                      writing <dir>/grad_f.bin and <dir>/grad_mu.bin
+//   bwd_cpu <dir> sched [blocks]  the field from the balanced schedule sized
+//                      for that many resident blocks (one piece per block,
+//                      each block locating its own)
 #include "backward_body.hpp"
 #include "cpu_attenuation.hpp"
 
@@ -96,19 +99,27 @@ int run_grad(const std::string &dir, const cpu_build::Problem &p, const std::vec
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc < 2 || argc > 3 || (argc == 3 && std::string(argv[2]) != "grad")) {
-    std::fprintf(stderr, "usage: bwd_cpu <dir> [grad]\n");
+  const std::string mode = argc >= 3 ? std::string(argv[2]) : std::string();
+  if (argc < 2 || argc > 4 || (argc >= 3 && mode != "grad" && mode != "sched") ||
+      (argc == 4 && mode != "sched")) {
+    std::fprintf(stderr, "usage: bwd_cpu <dir> [grad | sched [blocks]]\n");
     return 2;
   }
   const std::string dir = argv[1];
+  const bool sched_mode = mode == "sched";
+  const int sched_blocks = argc == 4 ? std::atoi(argv[3]) : 7;
   const cpu_build::Problem p = cpu_build::load_problem(dir);
   const Geo<float> &g = p.g;
   const int N = p.N, Nz = p.Nz, n = p.n, H = p.H;
   const auto proj = cpu_build::read_file<float>(dir + "/p.bin", (size_t)n * N * Nz);
   const size_t n_jext = g.je1 - g.je0;
   std::vector<float> W((size_t)n * p.P * n_jext * Nz, std::numeric_limits<float>::quiet_NaN());
-  if (argc == 3) return run_grad(dir, p, proj, W);
+  if (mode == "grad") return run_grad(dir, p, proj, W);
   const cpu_build::Attenuation att = cpu_build::build_attenuation(p);
+  if (sched_mode && att.att) {
+    std::fprintf(stderr, "the balanced schedule runs without attenuation or with the field\n");
+    return 2;
+  }
   const FieldJob<float> FJ{g, p.tables(), proj.data(), att.mu, att.att, att.att ? 1 : 0, att.attf,
                            p.sel.data(), n, W.data(), nullptr, nullptr, nullptr};
 
@@ -120,10 +131,31 @@ int main(int argc, char **argv) {
     return 2;
   }
   const int runs = (Nz + RZ - 1) / RZ;
+  // a scheduled piece holds up to kPieceMax planes, a chunk L
+  const size_t planes = sched_mode ? std::max(p.L, kPieceMax) : p.L;
   std::vector<unsigned char> smem(
-      ((size_t)kTJ * (runs * RZ + 2 * H + 1) + (size_t)p.L * 2 * (2 * H + 1)) * sizeof(float));
+      ((size_t)kTJ * (runs * RZ + 2 * H + 1) + planes * 2 * (2 * H + 1)) * sizeof(float));
+  const int npairs = n * g.ntiles;
+  const int grid = sched_grid_blocks(n, g.ntiles, g.P, sched_blocks);
+  int pieces = npairs * p.C;
+  if (sched_mode) {
+    // the schedule as block 0 of the forward writes it (for the printout)
+    std::vector<int> sched(2 + (size_t)npairs + 1, -1);
+    std::vector<unsigned char> none(64);
+    cpu_build::run_grid(1, 1, 1, bt, none, [&](int, int, int, unsigned char *) {
+      sched_locate<float>(g, p.sel.data(), p.tables().ranges, n, sched_blocks, 0, sched.data());
+    });
+    pieces = sched[0];
+  }
   auto run = [&](auto h_c) {
     constexpr int HH = decltype(h_c)::value;
+    if (sched_mode) {
+      cpu_build::run_grid(grid, 1, 1, bt, smem, [&](int b, int, int, unsigned char *s) {
+        if (att.attf) field_sched_body<float, RZ, QS, HH, true>(FJ, sched_blocks, b, s);
+        else field_sched_body<float, RZ, QS, HH, false>(FJ, sched_blocks, b, s);
+      });
+      return;
+    }
     cpu_build::run_grid(n, g.ntiles, p.C, bt, smem, [&](int al, int jt, int c, unsigned char *s) {
       if (att.attf) field_body<float, RZ, QS, HH, false, true>(FJ, al, jt, c, s);
       else field_body<float, RZ, QS, HH, false, false>(FJ, al, jt, c, s);
@@ -152,6 +184,7 @@ int main(int argc, char **argv) {
     }
     cpu_build::write_file(dir + (mode == 0 ? "/bp_ref.bin" : "/bp_adj.bin"), out);
   }
-  std::printf("H %d w_written %zu w_total %zu\n", H, w_written, W.size());
+  std::printf("H %d w_written %zu w_total %zu blocks %d pieces %d\n", H, w_written, W.size(),
+              sched_mode ? grid : npairs * p.C, pieces);
   return 0;
 }

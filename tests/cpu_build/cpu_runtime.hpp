@@ -43,6 +43,43 @@ int __syncthreads_or(int pred) {
   return r;
 }
 
+namespace {
+int g_shfl_slot[1024];
+}  // namespace
+
+int __builtin_amdgcn_update_dpp(int old, int src, int dpp_ctrl, int row_mask, int bank_mask,
+                                bool bound_ctrl) {
+  if (row_mask != 0xF || bank_mask != 0xF) {
+    std::fprintf(stderr, "DPP row/bank masks other than 0xF are not emulated\n");
+    std::abort();
+  }
+  const unsigned t = threadIdx.x, l = t & 15u, row0 = t & ~15u;
+  int from = -1;  // source lane, -1: outside the row
+  const int n = dpp_ctrl & 0xF;
+  if (dpp_ctrl < 0x100) from = (int)((t & ~3u) | ((unsigned)(dpp_ctrl >> (2 * (t & 3u))) & 3u));
+  else if (dpp_ctrl >= 0x101 && dpp_ctrl <= 0x10F) from = (int)l + n <= 15 ? (int)(t + n) : -1;
+  else if (dpp_ctrl >= 0x111 && dpp_ctrl <= 0x11F) from = (int)l >= n ? (int)(t - n) : -1;
+  else if (dpp_ctrl >= 0x121 && dpp_ctrl <= 0x12F) from = (int)(row0 + ((l - n) & 15u));
+  else {
+    std::fprintf(stderr, "DPP control 0x%x is not emulated\n", dpp_ctrl);
+    std::abort();
+  }
+  g_shfl_slot[t] = src;
+  cpu_block_barrier->arrive_and_wait();
+  const int r = from >= 0 ? g_shfl_slot[from] : (bound_ctrl ? 0 : old);
+  cpu_block_barrier->arrive_and_wait();
+  return r;
+}
+
+int __builtin_amdgcn_readlane(int v, int lane) {
+  const unsigned t = threadIdx.x;
+  g_shfl_slot[t] = v;
+  cpu_block_barrier->arrive_and_wait();
+  const int r = g_shfl_slot[(t & ~63u) + (unsigned)lane];
+  cpu_block_barrier->arrive_and_wait();
+  return r;
+}
+
 namespace cpu_build {
 
 template <typename T> std::vector<T> read_file(const std::string &path, size_t count) {

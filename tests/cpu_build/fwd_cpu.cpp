@@ -52,19 +52,13 @@ int main(int argc, char **argv) {
   }
   std::vector<int> sched(2 + (size_t)npairs + 1, -1);
   std::vector<unsigned char> smem(std::max(fwd_smem_bytes<float, RZ, 8>(Nz, p.L), (size_t)64));
-  if (sched_mode) {
-    // launch_fwd_sched_h: the one-block schedule kernel first
-    const int sbt = npairs >= 1024 ? 1024 : ((npairs + 63) / 64) * 64;
-    cpu_build::run_grid(1, 1, 1, sbt, smem, [&](int, int, int, unsigned char *) {
-      schedule_body<float>(g, p.sel.data(), p.tables().ranges, n, sched_blocks, sched.data());
-    });
-  }
   auto run = [&](auto h_c) {
     constexpr int HH = decltype(h_c)::value;
     if (sched_mode) {
       cpu_build::run_grid(grid, 1, 1, bt, smem, [&](int b, int, int, unsigned char *s) {
-        if (att.attf) fwd_sched_body<float, RZ, QS, HH, true>(J, p.tables().ranges, sched.data(), b, s);
-        else fwd_sched_body<float, RZ, QS, HH, false>(J, p.tables().ranges, sched.data(), b, s);
+        if (att.attf) fwd_sched_body<float, RZ, QS, HH, true>(J, p.tables().ranges, sched.data(), sched_blocks, b, s);
+        else fwd_sched_body<float, RZ, QS, HH, false>(J, p.tables().ranges, sched.data(), sched_blocks, b,
+                                                        s);
       });
       return;
     }

@@ -37,6 +37,16 @@ inline int __clzll(unsigned long long v) { return __builtin_clzll(v); }
 // consecutive calls never share a slot.
 int __syncthreads_or(int pred);
 inline float __expf(float v) { return std::exp(v); }
+// Cross-lane operations of 64-lane waves, emulated block-synchronously (two
+// barriers each, like __syncthreads_or): the bodies call them from control
+// flow that every lane of the block reaches.  DPP controls: quad_perm
+// (0x00-0xFF), row_shl (0x101-0x10F), row_shr (0x111-0x11F), row_ror
+// (0x121-0x12F); a source lane outside the row gives `old` (or 0 with
+// bound_ctrl).
+int __builtin_amdgcn_update_dpp(int old, int src, int dpp_ctrl, int row_mask, int bank_mask,
+                                bool bound_ctrl);
+int __builtin_amdgcn_readlane(int v, int lane);
+inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
 
 struct float2 {
   float x, y;

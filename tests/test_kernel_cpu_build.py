@@ -222,6 +222,33 @@ def test_backward_bodies_on_cpu_match_oracle(cpu_build, tmp_path, name, spec, se
     assert e_ref <= 1e-5 and e_adj <= 1e-5
 
 
+@pytest.mark.parametrize("blocks", [1, 6, 41])
+@pytest.mark.parametrize("name,spec,sel,att_mode", [c for c in CASES if c[3] != 2],
+                         ids=[c[0] for c in CASES if c[3] != 2])
+def test_balanced_field_schedule_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel, att_mode,
+                                                       blocks):
+    """The back-projection field from the balanced schedule (the forward's
+    plane ranges split into pieces for 1, 6 and 41 blocks), then both
+    gathers: every W entry a gather reads is written (W starts as NaN) and
+    the reference back projection and exact adjoint match the oracle."""
+    spec = dict(spec)
+    pa = problem_arrays(spec.pop("n"), spec.pop("n_angles"), seed=6, **spec)
+    _write_problem(tmp_path, pa, sel, att_mode=att_mode)
+    y = torch.rand((len(sel), pa["n"], pa["nz"]), generator=torch.Generator().manual_seed(7))
+    y.numpy().astype(np.float32).tofile(tmp_path / "p.bin")
+    stats = _exec(cpu_build["bwd_cpu"], tmp_path, ["sched", str(blocks)])
+    shape = (pa["n"], pa["n"], pa["nz"])
+    ref_cpu = torch.from_numpy(np.fromfile(tmp_path / "bp_ref.bin", dtype=np.float32).reshape(shape))
+    adj_cpu = torch.from_numpy(np.fromfile(tmp_path / "bp_adj.bin", dtype=np.float32).reshape(shape))
+    g = oracle_geometry(pa)
+    e_ref = rel_l2(ref_cpu, so.backward(g, y, torch.tensor(sel)))
+    e_adj = rel_l2(adj_cpu, so.exact_adjoint(g, y, torch.tensor(sel)))
+    print(f"{name}, {blocks} blocks: {stats['pieces']} pieces, reference rel-L2 {e_ref:.1e}, "
+          f"adjoint rel-L2 {e_adj:.1e}")
+    assert bool(torch.isfinite(ref_cpu).all()) and bool(torch.isfinite(adj_cpu).all())
+    assert e_ref <= 1e-5 and e_adj <= 1e-5
+
+
 @pytest.mark.skipif(os.environ.get("SPECT_CPU_BUILD_FULL") != "1",
                     reason="about 20 minutes: set SPECT_CPU_BUILD_FULL=1")
 def test_backward_bodies_on_cpu_plain_128(cpu_build, tmp_path):
