@@ -318,6 +318,43 @@ int spect_kem_apply(int dtype, const void *kernel, const void *f, void *out, int
 int spect_dvf_warp(int dtype, const void *f, const void *dvf, void *out, int32_t nx, int32_t ny,
                    int32_t nz, void *stream);
 
+/* ---- whole rotated frames: the per-angle projector path ----------------
+ * SPECTSystemMatrix.forward/backward (spect_system_matrix.py:145-156,
+ * 187-198) hand each angle's padded, rotated object to the obj2obj
+ * transforms.  With transforms other than attenuation followed by PSF (or a
+ * proj2proj transform) the projector runs that loop on these volumes,
+ * [n][P][P][nz] (P = padded width, axis 1 the ray direction towards the
+ * detector at +x after rotation).  n is 1..65535; entries of angle_idx out
+ * of range give zero volumes; arrays 16-byte aligned. */
+
+/* out[a] = RotationTransform.backward(pad_object(f), 270 - beta_a)
+ * (spect_system_matrix.py:149-151; transforms/shared/spatial.py:39-53) for
+ * beta_a = angles[angle_idx[a]]: the bilinear samples spect_forward takes.
+ * f [nx][ny][nz] (unpadded; zeros outside). */
+int spect_rotate_object(const spect_plan *p, int dtype, const void *f, const int32_t *angle_idx,
+                        int n, void *out, void *stream);
+
+/* out = unpad_object(sum_a RotationTransform.forward(vol[a], 270 - beta_a))
+ * (spect_system_matrix.py:194-200; transforms/shared/spatial.py:22-37), the
+ * rotation and sum of the reference back projection; vol [n][P][P][nz], out [nx][ny][nz] with out_mode
+ * SPECT_OUT_STORE or SPECT_OUT_ADD. */
+int spect_rotate_back(const spect_plan *p, int dtype, const void *vol, const int32_t *angle_idx,
+                      int n, int out_mode, void *out, void *stream);
+
+/* out[a] = get_prob_of_detection_matrix(rotate_detector_z(pad_object(mu),
+ * beta_a), dx) (attenuation.py:11-21, 88-91): exp(-(S - mu_rot*dx/2)) with
+ * the reverse cumsum S in FP64, the factors spect_att_field_build stores,
+ * over the whole frame.  mu [nx][ny][nz]. */
+int spect_att_volume(const spect_plan *p, int dtype, const void *mu, const int32_t *angle_idx,
+                     int n, void *out, void *stream);
+
+/* out[a] = SPECTPSFTransform.forward(vol[a], angle_idx[a]) (psf.py:199-214;
+ * Seperable1DBlurNet, psf.py:27-40): every plane x blurred along y with the
+ * plan's taps_r of that plane for the angle's radius, then along z with its
+ * taps_z, zeros outside the frame.  The plan must have a PSF; vol != out. */
+int spect_psf_volume(const spect_plan *p, int dtype, const void *vol, const int32_t *angle_idx,
+                     int n, void *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -175,11 +175,12 @@ def get_distance(lx: int, r: float, dx: float) -> np.ndarray:
 
 @dataclass
 class PSFModel:
-    """SPECTPSFMeta (metadata/SPECT/spect_metadata.py:66-94), gaussian only."""
+    """SPECTPSFMeta (metadata/SPECT/spect_metadata.py:66-94)."""
     sigma_fit_params: tuple
     sigma_fit: object = None  # default a*r+b (spect_metadata.py:79)
     kernel_dimensions: str = "2D"
     min_sigmas: float = 3
+    shape: str = "gaussian"   # or "square"
 
     def fit(self, d):
         if self.sigma_fit is None:
@@ -210,14 +211,26 @@ def gaussian_taps(sigmas: np.ndarray, k: int, dtype=torch.float32) -> torch.Tens
     return kern.to(dtype)[:, 0, :]
 
 
+def square_taps(sigmas: np.ndarray, k: int, dtype=torch.float32) -> torch.Tensor:
+    """get_1D_PSF_layer_square (psf.py:85-101): taps [len(sigmas), k]."""
+    n = len(sigmas)
+    x = torch.arange(-int(k // 2), int(k // 2) + 1).unsqueeze(0).unsqueeze(0).repeat((n, 1, 1))
+    s = torch.tensor(sigmas).to(dtype).reshape((n, 1, 1))
+    kern = 1 - torch.abs(x) / s
+    kern[kern < 0] = 0
+    kern = kern / kern.sum(axis=-1).unsqueeze(-1)
+    return kern.to(dtype)[:, 0, :]
+
+
 def psf_layers(psf: PSFModel, shape, dr, radius: float, dtype=torch.float32):
     """_configure_simple_model (psf.py:123-142) for one radius: (taps_r, taps_z|None)."""
     n = shape[0]
     kr = psf_kernel_size(psf, n, radius, dr[0], dr[0])
     kz = psf_kernel_size(psf, n, radius, dr[0], dr[2])
     sig = psf_sigma(psf, n, radius, dr[0])
-    taps_r = gaussian_taps(sig / dr[0], kr, dtype)
-    taps_z = gaussian_taps(sig / dr[2], kz, dtype) if psf.kernel_dimensions == "2D" else None
+    taps = square_taps if psf.shape == "square" else gaussian_taps
+    taps_r = taps(sig / dr[0], kr, dtype)
+    taps_z = taps(sig / dr[2], kz, dtype) if psf.kernel_dimensions == "2D" else None
     return taps_r, taps_z
 
 
@@ -293,10 +306,35 @@ def _obj2obj_backward(g: SPECTGeometry, obj_i, ang_idx):
     return obj_i
 
 
-def forward(g: SPECTGeometry, f: torch.Tensor, angle_indices=None) -> torch.Tensor:
-    """SPECTSystemMatrix.forward (spect_system_matrix.py:120-157)."""
+def _obj2obj_list(g: SPECTGeometry, obj2obj):
+    """The obj2obj transforms of a call: the geometry's (attenuation then
+    PSF) by default, else a list whose items are "att", "psf" or objects with
+    ``forward(obj_i, ang_idx)`` / ``backward(obj_i, ang_idx)``."""
+    if obj2obj is None:
+        return (["att"] if g.mu is not None else []) + (["psf"] if g.psf is not None else [])
+    return list(obj2obj)
+
+
+def _apply_obj2obj(g: SPECTGeometry, t, obj_i, ang_idx: int, backward: bool):
+    """attenuation.forward/backward (attenuation.py:72-115), psf.forward/
+    backward (psf.py:199-231), or a caller's transform."""
+    if t == "att":
+        return obj_i * attenuation_factor(g.mu, g.angles[ang_idx], g.dr[0])
+    if t == "psf":
+        tr, tz = g.layers(ang_idx)
+        return psf_apply(obj_i, tr, tz)
+    return t.backward(obj_i, ang_idx) if backward else t.forward(obj_i, ang_idx)
+
+
+def forward(g: SPECTGeometry, f: torch.Tensor, angle_indices=None, obj2obj=None,
+            proj2proj=()) -> torch.Tensor:
+    """SPECTSystemMatrix.forward (spect_system_matrix.py:120-157): per angle
+    pad, rotate, obj2obj transforms in order, sum along x; then the proj2proj
+    transforms (objects with ``forward(proj, angle_indices)``) on the padded
+    projections; unpad."""
     if angle_indices is None:
         angle_indices = torch.arange(g.n_angles)
+    ts = _obj2obj_list(g, obj2obj)
     f = f.to(g.dtype)
     p = compute_pad_size(g.shape[1])
     proj = torch.zeros((len(angle_indices), g.shape[1] + 2 * p, g.shape[2]), dtype=g.dtype)
@@ -304,27 +342,48 @@ def forward(g: SPECTGeometry, f: torch.Tensor, angle_indices=None) -> torch.Tens
         a = int(angle_indices[i])
         obj_i = pad_object(f)
         obj_i = rotate_xyz(obj_i, -(270 - g.angles[a]))       # RotationTransform.backward
-        obj_i = _obj2obj_forward(g, obj_i, a)
+        for t in ts:
+            obj_i = _apply_obj2obj(g, t, obj_i, a, backward=False)
         proj[i] = obj_i.sum(axis=0)
+    for t in proj2proj:
+        proj = t.forward(proj, angle_indices)
     return unpad_proj(proj)
 
 
-def backward(g: SPECTGeometry, proj: torch.Tensor, angle_indices=None) -> torch.Tensor:
+def backward(g: SPECTGeometry, proj: torch.Tensor, angle_indices=None, obj2obj=None,
+             proj2proj=()) -> torch.Tensor:
     """SPECTSystemMatrix.backward (spect_system_matrix.py:159-201): gather at
-    -theta (RotationTransform.forward with 270-beta), not the transpose."""
+    -theta (RotationTransform.forward with 270-beta), not the transpose; the
+    proj2proj and obj2obj transforms' backward in reverse order."""
     if angle_indices is None:
         angle_indices = torch.arange(g.n_angles)
+    ts = _obj2obj_list(g, obj2obj)
     box = pad_object(torch.ones(g.shape, dtype=g.dtype), mode="back_project")
     proj = pad_proj(proj.to(g.dtype))
+    for t in list(proj2proj)[::-1]:
+        proj = t.backward(proj, angle_indices)
     pp = compute_pad_size(g.shape[1])
     obj = torch.zeros((g.shape[0] + 2 * pp, g.shape[1] + 2 * pp, g.shape[2]), dtype=g.dtype)
     for i in range(len(angle_indices)):
         a = int(angle_indices[i])
         obj_i = proj[i].unsqueeze(0) * box
-        obj_i = _obj2obj_backward(g, obj_i, a)
+        for t in ts[::-1]:
+            obj_i = _apply_obj2obj(g, t, obj_i, a, backward=True)
         obj_i = rotate_xyz(obj_i, 270 - g.angles[a])          # RotationTransform.forward
         obj = obj + obj_i
     return unpad_object(obj)
+
+
+def average_prob_matrix(g: SPECTGeometry) -> torch.Tensor:
+    """SPECTAttenuationTransform.compute_average_prob_matrix
+    (attenuation.py:117-124): each angle's probability of detection rotated
+    back (rotate_detector_z negative), averaged over the angles, unpadded."""
+    mu = pad_object(g.mu)
+    acc = torch.zeros(mu.shape, dtype=g.dtype)
+    for a in range(g.n_angles):
+        prob = prob_of_detection(rotate_detector_z(mu, g.angles[a]), g.dr[0])
+        acc = acc + rotate_detector_z(prob, g.angles[a], negative=True)
+    return unpad_object(acc / g.n_angles)
 
 
 def forward_autograd(g: SPECTGeometry, f: torch.Tensor, mu: torch.Tensor | None,

@@ -33,6 +33,7 @@ EXPORTED = (
     "spect_poisson_ratio", "spect_rdp_gradient", "spect_rdp_hessian_vjp", "spect_rdp_hessian_jvp",
     "spect_bsrem_update", "spect_ew_scatter", "spect_smooth_axis",
     "spect_kem_apply", "spect_dvf_warp",
+    "spect_rotate_object", "spect_rotate_back", "spect_att_volume", "spect_psf_volume",
 )
 
 
@@ -137,6 +138,11 @@ def _declare(lib):
     lib.spect_dvf_warp.argtypes = [ci, vp, vp, vp, i32, i32, i32, vp]
     lib.spect_bsrem_update.restype = ci
     lib.spect_bsrem_update.argtypes = [ci, vp, vp, vp, vp, vp, i64, dbl, dbl, dbl, dbl, vp]
+    for name in ("spect_rotate_object", "spect_att_volume", "spect_psf_volume"):
+        getattr(lib, name).restype = ci
+        getattr(lib, name).argtypes = [vp, ci, vp, vp, ci, vp, vp]
+    lib.spect_rotate_back.restype = ci
+    lib.spect_rotate_back.argtypes = [vp, ci, vp, vp, ci, ci, vp, vp]
 
 
 def lib():

@@ -161,6 +161,7 @@ class LinearPreconditionedGradientAscentAlgorithm(PreconditionedGradientAscentAl
 def _fused_ok(alg) -> bool:
     lk = alg.likelihood
     return (isinstance(lk, PoissonLogLikelihood) and isinstance(lk.system_matrix, SPECTSystemMatrix)
+            and lk.system_matrix.fused
             and alg.object_prediction.device.type == "cuda"
             and alg.object_prediction.dtype in (torch.float32, torch.float64)
             and alg.object_prediction.is_contiguous())

@@ -43,6 +43,7 @@ class BatchedOSEM:
         lib = _lib.lib()
         for i, lk in enumerate(likelihoods):
             sm = lk.system_matrix
+            sm._require_fused("batched OSEM")
             device = lk.projections.device
             if device.type != "cuda":
                 raise _lib.SpectLibraryError("batched OSEM needs device tensors")

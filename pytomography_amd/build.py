@@ -16,7 +16,7 @@ import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["forward.hip", "backward.hip", "misc.hip", "scatter.hip", "shared.hip", "capi.hip"]
+SOURCES = ["forward.hip", "backward.hip", "misc.hip", "scatter.hip", "shared.hip", "volume.hip", "capi.hip"]
 OUT = os.path.join(HERE, "libspect_hip.so")
 ARCH = os.environ.get("SPECT_OFFLOAD_ARCH", "gfx950")
 

@@ -1176,6 +1176,96 @@ int spect_kem_apply(int dtype, const void *kernel, const void *f, void *out, int
   return SPECT_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The plan's geometry over the full frame width: every rotated-frame column
+// [0, P), not only those the fused projectors visit.
+template <typename R> Geo<R> geo_full(const spect_plan *p) {
+  Geo<R> g = geo<R>(p);
+  g.je0 = 0;
+  g.je1 = p->P;
+  g.ntiles = (p->P + kTJ - 1) / kTJ;
+  return g;
+}
+
+int check_volume_call(const spect_plan *p, int dtype, const int32_t *angle_idx, int n,
+                      std::initializer_list<const void *> arrays) {
+  if (!p) return fail(SPECT_E_ARG, "null plan");
+  if (dtype != SPECT_F32 && dtype != SPECT_F64) return fail(SPECT_E_ARG, "dtype must be F32 or F64");
+  if (dtype == SPECT_F64 && !p->has64) return fail(SPECT_E_ARG, "plan has no float64 tables");
+  if (!angle_idx) return fail(SPECT_E_ARG, "null angle_idx");
+  if (n <= 0 || n > 65535) return fail(SPECT_E_ARG, "angle count must be 1..65535");
+  for (const void *a : arrays)
+    if (!a) return fail(SPECT_E_ARG, "null array argument");
+  return check_arrays(arrays);
+}
+
+}  // namespace
+
+extern "C" {
+
+int spect_rotate_object(const spect_plan *p, int dtype, const void *f, const int32_t *angle_idx,
+                        int n, void *out, void *stream) {
+  if (int rc = check_volume_call(p, dtype, angle_idx, n, {f, out})) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_rotate<float>(geo_full<float>(p), tables<float>(p), (const float *)f, angle_idx, n,
+                                 (float *)out, st));
+  else
+    HIP_TRY(launch_rotate<double>(geo_full<double>(p), tables<double>(p), (const double *)f, angle_idx,
+                                  n, (double *)out, st));
+  return SPECT_OK;
+}
+
+int spect_rotate_back(const spect_plan *p, int dtype, const void *vol, const int32_t *angle_idx, int n,
+                      int out_mode, void *out, void *stream) {
+  if (int rc = check_volume_call(p, dtype, angle_idx, n, {vol, out})) return rc;
+  if (out_mode != SPECT_OUT_STORE && out_mode != SPECT_OUT_ADD)
+    return fail(SPECT_E_ARG, "out_mode must be SPECT_OUT_STORE or SPECT_OUT_ADD");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  // the reference back projection's gather (mode 0) over full-width frames
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_bp_gather<float>(geo_full<float>(p), tables<float>(p), (const float *)vol, angle_idx,
+                                    n, 0, out_mode, (float *)out, nullptr, nullptr, 0.0f, 0.0f,
+                                    nullptr, nullptr, nullptr, st));
+  else
+    HIP_TRY(launch_bp_gather<double>(geo_full<double>(p), tables<double>(p), (const double *)vol,
+                                     angle_idx, n, 0, out_mode, (double *)out, nullptr, nullptr, 0.0,
+                                     0.0, nullptr, nullptr, nullptr, st));
+  return SPECT_OK;
+}
+
+int spect_att_volume(const spect_plan *p, int dtype, const void *mu, const int32_t *angle_idx, int n,
+                     void *out, void *stream) {
+  if (int rc = check_volume_call(p, dtype, angle_idx, n, {mu, out})) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  // the attenuation-field body over full-width frames
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_att_field<float>(geo_full<float>(p), tables<float>(p), (const float *)mu, angle_idx,
+                                    n, (float *)out, st));
+  else
+    HIP_TRY(launch_att_field<double>(geo_full<double>(p), tables<double>(p), (const double *)mu,
+                                     angle_idx, n, (double *)out, st));
+  return SPECT_OK;
+}
+
+int spect_psf_volume(const spect_plan *p, int dtype, const void *vol, const int32_t *angle_idx, int n,
+                     void *out, void *stream) {
+  if (int rc = check_volume_call(p, dtype, angle_idx, n, {vol, out})) return rc;
+  if (p->H <= 0) return fail(SPECT_E_ARG, "the plan has no PSF");
+  if (vol == out) return fail(SPECT_E_ARG, "vol and out must differ");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == SPECT_F32)
+    HIP_TRY(launch_psf_volume<float>(geo_full<float>(p), tables<float>(p), (const float *)vol, angle_idx,
+                                     n, (float *)out, st));
+  else
+    HIP_TRY(launch_psf_volume<double>(geo_full<double>(p), tables<double>(p), (const double *)vol,
+                                      angle_idx, n, (double *)out, st));
+  return SPECT_OK;
+}
+
 int spect_dvf_warp(int dtype, const void *f, const void *dvf, void *out, int32_t nx, int32_t ny,
                    int32_t nz, void *stream) {
   if (!f || !dvf || !out) return fail(SPECT_E_ARG, "bad DVF arguments");

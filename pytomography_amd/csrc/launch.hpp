@@ -212,4 +212,15 @@ hipError_t launch_kem(const R *K, const R *f, R *out, int nx, int ny, int nz, in
 template <typename R>
 hipError_t launch_dvf(const R *f, const R *dvf, R *out, int nx, int ny, int nz, hipStream_t st);
 
+// Whole rotated-frame volumes for the per-angle projector path (volume.hip):
+// out[n][P][P][Nz] = the object sampled on the rotated frame of each angle;
+// the PSF blur of such volumes (g.H > 0), out != vol.
+template <typename R>
+hipError_t launch_rotate(const Geo<R> &g, const Tables<R> &tb, const R *f, const int32_t *angle_idx,
+                         int n, R *out, hipStream_t st);
+template <typename R> size_t psf_volume_smem_bytes(const Geo<R> &g);
+template <typename R>
+hipError_t launch_psf_volume(const Geo<R> &g, const Tables<R> &tb, const R *vol,
+                             const int32_t *angle_idx, int n, R *out, hipStream_t st);
+
 }  // namespace spect

@@ -18,44 +18,29 @@ Extras beyond the reference:
 """
 from __future__ import annotations
 
-import ctypes
+import inspect
 
-import numpy as np
 import torch
 
 import pytomography_amd
 from pytomography_amd import _lib
 from pytomography_amd.transforms.SPECT import SPECTAttenuationTransform, SPECTPSFTransform
-from pytomography_amd.utils import compute_pad_size
+from pytomography_amd.utils import compute_pad_size, pad_proj, unpad_proj
 from ..system_matrix import SystemMatrix
+from .plan import RotatedFrames, SpectPlan, plan_geometry
 
 
-class _Plan:
-    """Owns one ``spect_plan`` (device tables) for one device."""
-
-    def __init__(self, geom_args: dict, device: torch.device):
-        self.lib = _lib.lib()
-        self._keep = geom_args["_keep"]  # host arrays referenced by the geom struct
-        g = _lib.SpectGeom(**{k: v for k, v in geom_args.items() if k != "_keep"})
-        handle = ctypes.c_void_p()
-        with torch.cuda.device(device):
-            _lib.check(self.lib.spect_plan_create(ctypes.byref(g), ctypes.byref(handle)))
-        self.handle = handle
-        dims = [ctypes.c_int32() for _ in range(4)]
-        _lib.check(self.lib.spect_plan_dims(handle, *[ctypes.byref(d) for d in dims]))
-        self.padded, self.pad, self.psf_half, self.chunks = (d.value for d in dims)
-
-    def __del__(self):
-        try:
-            if getattr(self, "handle", None) is not None and self.handle.value:
-                self.lib.spect_plan_destroy(self.handle)
-                self.handle = None
-        except Exception:  # pragma: no cover - interpreter shutdown
-            pass
-
-
-def _as_ptr(arr: np.ndarray, ctype):
-    return arr.ctypes.data_as(ctypes.POINTER(ctype))
+def _accepts_angles(transform) -> bool:
+    """Whether a proj2proj transform's forward takes the projections' angle
+    indices after the projections (CutOffTransform's signature,
+    transforms/SPECT/cutoff.py:9-20) or the projections alone (the call of
+    spect_system_matrix.py:155-156)."""
+    try:
+        params = list(inspect.signature(transform.forward).parameters.values())
+    except (TypeError, ValueError):
+        return False
+    positional = [p for p in params if p.kind in (p.POSITIONAL_ONLY, p.POSITIONAL_OR_KEYWORD)]
+    return len(positional) >= 2 or any(p.kind == p.VAR_POSITIONAL for p in params)
 
 
 class SPECTSystemMatrix(SystemMatrix):
@@ -117,77 +102,33 @@ class SPECTSystemMatrix(SystemMatrix):
 
     # ------------------------------------------------------------------ set-up
     def _classify_transforms(self) -> None:
-        self._att = None
-        self._psf = None
-        for t in self.obj2obj_transforms:
-            if isinstance(t, SPECTAttenuationTransform):
-                if self._att is not None or self._psf is not None:
-                    raise NotImplementedError(
-                        "the fused kernels apply attenuation before the PSF; put "
-                        "SPECTAttenuationTransform first and only once")
-                self._att = t
-            elif isinstance(t, SPECTPSFTransform):
-                if self._psf is not None:
-                    raise NotImplementedError("only one SPECTPSFTransform is supported")
-                self._psf = t
-            else:
-                raise NotImplementedError(
-                    f"obj2obj transform {type(t).__name__} is not on the accelerated path")
-        if self.proj2proj_transforms:
-            raise NotImplementedError("proj2proj transforms are not on the accelerated path")
+        """Fused path: no proj2proj transform and obj2obj transforms
+        [SPECTAttenuationTransform?, SPECTPSFTransform?] in that order (a
+        Gaussian or square PSF from ``psf_meta``).  Anything else runs the
+        reference's per-angle loop over whole rotated frames."""
+        ts = self.obj2obj_transforms
+        kinds = ["att" if isinstance(t, SPECTAttenuationTransform)
+                 else "psf" if isinstance(t, SPECTPSFTransform) and t.fusable else "other" for t in ts]
+        self.fused = not self.proj2proj_transforms and kinds in ([], ["att"], ["psf"], ["att", "psf"])
+        self._att = ts[kinds.index("att")] if self.fused and "att" in kinds else None
+        self._psf = ts[kinds.index("psf")] if self.fused and "psf" in kinds else None
+        self._frames = None if self.fused else RotatedFrames(self.object_meta, self.proj_meta)
+        self._proj2proj_takes_angles = [_accepts_angles(t) for t in self.proj2proj_transforms]
+
+    def _require_fused(self, what: str) -> None:
+        if not self.fused:
+            raise NotImplementedError(
+                f"{what} needs the fused transforms (SPECTAttenuationTransform then SPECTPSFTransform, "
+                "no proj2proj transform); this system matrix runs the per-angle path")
 
     def _geom_args(self) -> dict:
-        nx, ny, nz = self._shape
-        angles = self.proj_meta.angles.detach().cpu()
-        a32 = np.ascontiguousarray(angles.to(torch.float32).numpy())
-        a64 = np.ascontiguousarray(angles.to(torch.float64).numpy())
-        keep = [a32, a64]
-        args = dict(nx=nx, ny=ny, nz=nz, n_angles=len(a32), dx=float(self.object_meta.dx),
-                    dy=float(self.object_meta.dy), dz=float(self.object_meta.dz),
-                    angles_deg=_as_ptr(a32, ctypes.c_float),
-                    angles_deg64=_as_ptr(a64, ctypes.c_double),
-                    psf_kr=0, psf_kz=0, n_radii=0, chunk_planes=self.chunk_planes,
-                    sched_rounds=self.sched_rounds)
-        if self._psf is not None:
-            radii = np.asarray(self.proj_meta.radii, dtype=np.float64).reshape(-1)
-            if radii.size == 1:
-                radii = np.full(len(a32), radii.item())
-            uniq = np.unique(radii)
-            ridx = np.ascontiguousarray(np.searchsorted(uniq, radii).astype(np.int32))
-            t32 = [self._psf.radius_taps(r, torch.float32) for r in uniq]
-            t64 = [self._psf.radius_taps(r, torch.float64) for r in uniq]
-            kr = max(t[0].shape[1] for t in t32)
-            two_d = t32[0][1] is not None
-            kz = max(t[1].shape[1] for t in t32) if two_d else 0
+        return plan_geometry(self.object_meta, self.proj_meta, self._psf, self.chunk_planes,
+                             self.sched_rounds)
 
-            def stack(tabs, k, which, npdt):
-                out = []
-                for t in tabs:
-                    w = t[which].numpy().astype(npdt)
-                    off = (k - w.shape[1]) // 2  # centre smaller kernels in zero taps
-                    full = np.zeros((w.shape[0], k), dtype=npdt)
-                    full[:, off:off + w.shape[1]] = w
-                    out.append(full)
-                return np.ascontiguousarray(np.stack(out))
-
-            tr32 = stack(t32, kr, 0, np.float32)
-            tr64 = stack(t64, kr, 0, np.float64)
-            keep += [ridx, tr32, tr64]
-            args.update(psf_kr=kr, n_radii=len(uniq), radius_index=_as_ptr(ridx, ctypes.c_int32),
-                        taps_r=_as_ptr(tr32, ctypes.c_float), taps_r64=_as_ptr(tr64, ctypes.c_double))
-            if two_d:
-                tz32 = stack(t32, kz, 1, np.float32)
-                tz64 = stack(t64, kz, 1, np.float64)
-                keep += [tz32, tz64]
-                args.update(psf_kz=kz, taps_z=_as_ptr(tz32, ctypes.c_float),
-                            taps_z64=_as_ptr(tz64, ctypes.c_double))
-        args["_keep"] = keep
-        return args
-
-    def _plan(self, device: torch.device) -> _Plan:
+    def _plan(self, device: torch.device) -> SpectPlan:
         key = (device.type, device.index)
         if key not in self._plans:
-            self._plans[key] = _Plan(self._geom_args(), device)
+            self._plans[key] = SpectPlan(self._geom_args(), device)
         return self._plans[key]
 
     def _workspace(self, device, dt: torch.dtype, n: int):
@@ -289,19 +230,101 @@ class SPECTSystemMatrix(SystemMatrix):
         device = pytomography_amd.device
         n = self.proj_meta.num_projections if subset_idx is None else len(self.subset_indices_array[subset_idx])
         ones = torch.ones((n, *self.proj_meta.shape[1:]), dtype=self.sensitivity_dtype, device=device)
+        if not self.fused:
+            return self._backward_frames(ones, subset_idx).to(pytomography_amd.dtype)
         out = self._backward(ones, subset_idx, _lib.BP_REFERENCE)
         return out.to(pytomography_amd.dtype)
 
     # ------------------------------------------------------------ projections
     def forward(self, object: torch.Tensor, subset_idx: int | None = None) -> torch.Tensor:
         """H_m f: projections [n, Ly, Lz] (spect_system_matrix.py:120-157)."""
+        if not self.fused:
+            return self._forward_frames(object, subset_idx)
         return self._project(object, subset_idx, "spect_forward")
+
+    # ------------------------------------------------------- per-angle path
+    def _angle_list(self, subset_idx):
+        """The angle indices of a call as the reference's loop sees them (a
+        long tensor on the device, spect_system_matrix.py:134-137)."""
+        if subset_idx is None:
+            return torch.arange(self.proj_meta.num_projections).to(pytomography_amd.device)
+        return self.subset_indices_array[subset_idx]
+
+    def _frame_chunk(self, plan, dt) -> int:
+        """Angles whose rotated frames are held at once (about 1 GiB)."""
+        per = plan.padded * plan.padded * self._shape[2] * (8 if dt == torch.float64 else 4)
+        return max(1, (1 << 30) // per)
+
+    def _proj2proj(self, proj, angles, backward: bool):
+        ts = self.proj2proj_transforms
+        order = range(len(ts) - 1, -1, -1) if backward else range(len(ts))
+        for k in order:
+            fn = ts[k].backward if backward else ts[k].forward
+            proj = fn(proj, angles) if self._proj2proj_takes_angles[k] else fn(proj)
+        return proj
+
+    def _forward_frames(self, object: torch.Tensor, subset_idx) -> torch.Tensor:
+        """spect_system_matrix.py:145-157: every angle's padded object rotated
+        (HIP), the obj2obj transforms in order, the sum along the rays, then
+        the proj2proj transforms on the padded projections."""
+        device = pytomography_amd.device if object.device.type != "cuda" else object.device
+        dt = self._compute_dtype(object)
+        f = _lib.require_device_tensor(object.to(device), "object", dt)
+        if tuple(f.shape) != self._shape:
+            raise ValueError(f"object shape {tuple(f.shape)} != {self._shape}")
+        angles = self._angle_list(subset_idx)
+        n = len(angles)
+        plan = self._frames.plan(device)
+        proj = torch.empty((n, plan.padded, self._shape[2]), dtype=dt, device=device)
+        chunk = self._frame_chunk(plan, dt)
+        for c0 in range(0, n, chunk):
+            frames = self._frames.rotate(f, angles[c0:c0 + chunk])
+            for k in range(frames.shape[0]):
+                object_i = frames[k]
+                for transform in self.obj2obj_transforms:
+                    object_i = transform.forward(object_i, angles[c0 + k])
+                proj[c0 + k] = object_i.sum(axis=0)
+            del frames
+        proj = self._proj2proj(proj, angles, backward=False)
+        return unpad_proj(proj)
+
+    def _backward_frames(self, proj: torch.Tensor, subset_idx) -> torch.Tensor:
+        """spect_system_matrix.py:178-201: the proj2proj transforms in reverse
+        on the padded projections, each angle's projection smeared along the
+        rays inside the back-projection box (object columns only), the
+        obj2obj transforms in reverse, then the rotation and sum (HIP)."""
+        device = pytomography_amd.device if proj.device.type != "cuda" else proj.device
+        dt = self._compute_dtype(proj)
+        angles = self._angle_list(subset_idx)
+        n = len(angles)
+        p = proj.to(device=device, dtype=dt)
+        if tuple(p.shape) != (n, self._shape[1], self._shape[2]):
+            raise ValueError(f"projection shape {tuple(p.shape)} != {(n, self._shape[1], self._shape[2])}")
+        p = self._proj2proj(pad_proj(p), angles, backward=True)
+        plan = self._frames.plan(device)
+        P = plan.padded
+        box = torch.zeros((P, 1), dtype=dt, device=device)  # pad_object(ones, 'back_project')
+        box[plan.pad:plan.pad + self._shape[1]] = 1
+        out = None
+        chunk = self._frame_chunk(plan, dt)
+        for c0 in range(0, n, chunk):
+            m = min(chunk, n - c0)
+            frames = torch.empty((m, P, P, self._shape[2]), dtype=dt, device=device)
+            for k in range(m):
+                object_i = (p[c0 + k] * box).unsqueeze(0).expand(P, P, self._shape[2]).contiguous()
+                for transform in self.obj2obj_transforms[::-1]:
+                    object_i = transform.backward(object_i, angles[c0 + k])
+                frames[k] = object_i
+            out = self._frames.rotate_back(frames, angles[c0:c0 + m], out)
+            del frames
+        return out
 
     def backward_transpose(self, object: torch.Tensor, subset_idx: int | None = None) -> torch.Tensor:
         """B_m^T v, the transpose of the reference back projection ``backward``
         (spect_system_matrix.py:159-201): the forward chain with the rotation
         replaced by the transpose of the gather at 270-beta.  Used to
         differentiate through iterations that call ``backward``."""
+        self._require_fused("backward_transpose")
         return self._project(object, subset_idx, "spect_backward_transpose")
 
     def _project(self, object: torch.Tensor, subset_idx, entry: str) -> torch.Tensor:
@@ -348,10 +371,13 @@ class SPECTSystemMatrix(SystemMatrix):
     def backward(self, proj: torch.Tensor, subset_idx: int | None = None) -> torch.Tensor:
         """Reference back projection (spect_system_matrix.py:159-201): PSF,
         attenuation, then rotation by 270-beta (a bilinear gather)."""
+        if not self.fused:
+            return self._backward_frames(proj, subset_idx)
         return self._backward(proj, subset_idx, _lib.BP_REFERENCE)
 
     def backward_adjoint(self, proj: torch.Tensor, subset_idx: int | None = None) -> torch.Tensor:
         """Exact transpose H_m^T of ``forward``: <H f, y> = <f, H^T y>."""
+        self._require_fused("backward_adjoint")
         return self._backward(proj, subset_idx, _lib.BP_ADJOINT)
 
     def osem_step(self, object: torch.Tensor, projections: torch.Tensor, additive: torch.Tensor | None,
@@ -361,6 +387,7 @@ class SPECTSystemMatrix(SystemMatrix):
         tensor): pred = H_m f + s; r = g / (pred + delta);
         f += f/(n+delta) * (B_m r - n); values <= floor_value become floor_value.
         Returns pred (``projections_predicted``) unless keep_prediction is False."""
+        self._require_fused("osem_step")
         device = object.device
         dt = object.dtype
         if object.device.type != "cuda" or not object.is_contiguous() or object.data_ptr() % 16:
@@ -388,6 +415,7 @@ class SPECTSystemMatrix(SystemMatrix):
                         subset_idx: int | None = None) -> torch.Tensor:
         """H(mu) f with an explicit attenuation map (rows rebuilt every call, no
         cache): the forward of the differentiable projector."""
+        self._require_fused("forward_with_mu")
         device = object.device
         dt = self._compute_dtype(object)
         f = _lib.require_device_tensor(object, "object", dt)
@@ -407,6 +435,7 @@ class SPECTSystemMatrix(SystemMatrix):
     def forward_vjp(self, grad_proj: torch.Tensor, object: torch.Tensor, mu: torch.Tensor | None,
                     subset_idx: int | None = None, need_mu: bool = True):
         """(H(mu)^T u, (d H(mu) f / d mu)^T u) for upstream gradient u = grad_proj."""
+        self._require_fused("forward_vjp")
         device = object.device
         dt = self._compute_dtype(object)
         u = _lib.require_device_tensor(grad_proj, "grad_proj", dt)

@@ -1,9 +1,10 @@
 """SPECT attenuation transform (transforms/SPECT/attenuation.py:23-124).
 
-Inside ``SPECTSystemMatrix`` the attenuation factor
-exp(-(sum_{i'>=i} mu_rot dx - mu_rot(i) dx / 2)) is computed on the fly by
-the fused HIP kernels (rotation of mu, FP64 reverse cumsum, exp); this class
-carries the attenuation map and validates it.
+Inside ``SPECTSystemMatrix``'s fused kernels the attenuation factor
+exp(-(sum_{i'>=i} mu_rot dx - mu_rot(i) dx / 2)) is computed on the fly
+(rotation of mu, FP64 reverse cumsum, exp).  ``forward``/``backward`` apply
+the same factors to one angle's padded, rotated object for the per-angle
+projector path and for callers that use the transform on its own.
 """
 from __future__ import annotations
 
@@ -48,9 +49,45 @@ class SPECTAttenuationTransform(Transform):
         if tuple(self.attenuation_map.shape) != tuple(object_meta.shape):
             raise ValueError(f"attenuation map shape {tuple(self.attenuation_map.shape)} does not "
                              f"match the object shape {tuple(object_meta.shape)}")
+        self._frames = None
 
-    def forward(self, object_i, ang_idx):
-        raise NotImplementedError(
-            "SPECTAttenuationTransform is applied inside SPECTSystemMatrix's fused kernels")
+    def _rotated_frames(self):
+        if getattr(self, "_frames", None) is None:
+            from pytomography_amd.projectors.SPECT.plan import RotatedFrames
+            self._frames = RotatedFrames(self.object_meta, self.proj_meta)
+        return self._frames
 
-    backward = forward
+    @torch.no_grad()
+    def forward(self, object_i: torch.Tensor, ang_idx) -> torch.Tensor:
+        """attenuation.py:72-92: ``object_i`` [P, P, Lz] (padded and rotated so
+        the detector of angle ``ang_idx`` sits at +x) times that angle's
+        probability of detection (HIP kernel: rotated mu, FP64 reverse cumsum
+        with half the own voxel, exp).  With a vector of angles, ``object_i``
+        is [n, P, P, Lz]."""
+        dt = torch.float64 if object_i.dtype == torch.float64 else torch.float32
+        factor = self._rotated_frames().attenuation(self.attenuation_map.to(dt), ang_idx)
+        if object_i.dim() == 3:
+            factor = factor[0]
+        return object_i * factor
+
+    @torch.no_grad()
+    def backward(self, object_i: torch.Tensor, ang_idx) -> torch.Tensor:
+        """attenuation.py:94-115: the factor is diagonal, so the same product."""
+        return self.forward(object_i, ang_idx)
+
+    @torch.no_grad()
+    def compute_average_prob_matrix(self) -> torch.Tensor:
+        """attenuation.py:117-124: the probability of detection of every angle
+        rotated back to the object frame, averaged over the angles, unpadded."""
+        frames = self._rotated_frames()
+        dt = pytomography_amd.dtype
+        mu = self.attenuation_map.to(dt)
+        n = self.proj_meta.num_projections
+        plan = frames.plan(mu.device if mu.device.type == "cuda" else pytomography_amd.device)
+        per = plan.padded * plan.padded * self.object_meta.shape[2] * (8 if dt == torch.float64 else 4)
+        chunk = max(1, (1 << 30) // per)
+        out = None
+        for c0 in range(0, n, chunk):
+            idx = torch.arange(c0, min(n, c0 + chunk))
+            out = frames.rotate_back(frames.attenuation(mu, idx), idx, out)
+        return out / n

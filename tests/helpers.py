@@ -16,14 +16,19 @@ def problem_arrays(n, n_angles, seed=0, nz=None, att=False, psf=False):
                 radii=np.full(n_angles, synthetic.RADIUS))
 
 
-def oracle_geometry(pa, dtype=torch.float32):
-    psf = so.PSFModel(synthetic.PSF_PARAMS) if pa["psf"] else None
+def oracle_geometry(pa, dtype=torch.float32, psf_shape="gaussian", psf_dims="2D"):
+    psf = (so.PSFModel(synthetic.PSF_PARAMS, kernel_dimensions=psf_dims, shape=psf_shape,
+                       min_sigmas=1 if psf_shape == "square" else 3) if pa["psf"] else None)
     mu = None if pa["mu"] is None else torch.from_numpy(pa["mu"])
     return so.make_geometry((pa["n"], pa["n"], pa["nz"]), (synthetic.DX,) * 3, pa["angles"],
                             radii=pa["radii"], mu=mu, psf=psf, dtype=dtype)
 
 
-def product_system_matrix(pa, device="cuda", chunk_planes=0, attenuation_field=None):
+def product_system_matrix(pa, device="cuda", chunk_planes=0, attenuation_field=None,
+                          obj2obj=None, proj2proj=(), psf_shape="gaussian", psf_dims="2D"):
+    """The product system matrix of a problem: obj2obj transforms
+    [attenuation?, PSF?] by default, else ``obj2obj`` whose items are "att",
+    "psf" or transform objects, in that order."""
     import pytomography_amd
     from pytomography_amd.metadata.SPECT import SPECTObjectMeta, SPECTProjMeta, SPECTPSFMeta
     from pytomography_amd.transforms.SPECT import SPECTAttenuationTransform, SPECTPSFTransform
@@ -32,12 +37,18 @@ def product_system_matrix(pa, device="cuda", chunk_planes=0, attenuation_field=N
     om = SPECTObjectMeta((synthetic.DX,) * 3, (pa["n"], pa["n"], pa["nz"]))
     pm = SPECTProjMeta((pa["n"], pa["nz"]), (synthetic.DX, synthetic.DX), list(pa["angles"]),
                        radii=list(pa["radii"]))
+    if obj2obj is None:
+        obj2obj = (["att"] if pa["mu"] is not None else []) + (["psf"] if pa["psf"] else [])
     tr = []
-    if pa["mu"] is not None:
-        tr.append(SPECTAttenuationTransform(torch.from_numpy(pa["mu"]).to(device)))
-    if pa["psf"]:
-        tr.append(SPECTPSFTransform(SPECTPSFMeta(synthetic.PSF_PARAMS)))
-    return SPECTSystemMatrix(tr, [], om, pm, chunk_planes=chunk_planes,
+    for t in obj2obj:
+        if t == "att":
+            tr.append(SPECTAttenuationTransform(torch.from_numpy(pa["mu"]).to(device)))
+        elif t == "psf":
+            tr.append(SPECTPSFTransform(SPECTPSFMeta(synthetic.PSF_PARAMS, kernel_dimensions=psf_dims,
+                                                     shape=psf_shape)))
+        else:
+            tr.append(t)
+    return SPECTSystemMatrix(tr, list(proj2proj), om, pm, chunk_planes=chunk_planes,
                              attenuation_field=attenuation_field)
 
 

@@ -132,10 +132,19 @@ def test_unrolled_bsrem_gradient_matches_oracle_autograd(cuda):
         assert ex <= tol_x and eg <= tol_g and ef <= tol_g and eb <= tol_g
 
 
-def test_unrolled_bsrem_fp32_gradient_vs_fp64(cuda):
+@pytest.mark.parametrize("n,n_angles,iters,bound", [(48, 40, 4, 1e-3), (96, 90, 10, 1e-4)],
+                         ids=["48cube_4it", "96cube_10it"])
+def test_unrolled_bsrem_fp32_gradient_vs_fp64(cuda, n, n_angles, iters, bound):
+    """float32 against float64 for x_K and the gradients with respect to the
+    projections, x_0 and beta (attenuation + PSF, 4 subsets).  At 192^3 x 180
+    angles x 10 iterations d/dx_0 differs by about 2e-3 in float32, while
+    float64 runs whose data or x_0 are perturbed by 2^-24 (float32's
+    rounding) move it by at most 3e-5: the drift is rounding accumulated
+    through the 40 sub-iterations' forward and reverse passes, not the
+    conditioning of the inputs."""
     from pytomography_amd.priors import RelativeDifferencePrior
     from pytomography_amd.unrolled import UnrolledBSREM
-    pa = problem_arrays(48, 40, seed=39, att=True, psf=True)
+    pa = problem_arrays(n, n_angles, seed=39, att=True, psf=True)
     sm = product_system_matrix(pa)
     f_true = torch.from_numpy(pa["f"]).double().to(cuda)
     with torch.no_grad():
@@ -145,14 +154,16 @@ def test_unrolled_bsrem_fp32_gradient_vs_fp64(cuda):
     grads = {}
     for dt in (torch.float64, torch.float32):
         un = UnrolledBSREM(sm, RelativeDifferencePrior(beta=0.1, gamma=2.0), additive_term=s.to(dt),
-                           n_iters=4, n_subsets=4)
+                           n_iters=iters, n_subsets=4)
         d = data.to(dt).detach().clone().requires_grad_(True)
+        x0 = torch.ones(sm._shape, dtype=dt, device=cuda).requires_grad_(True)
         b = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
-        xk = un(d, None, b)
+        xk = un(d, x0, b)
         ((xk - f_true.to(dt)) ** 2).sum().backward()
-        grads[dt] = (xk.detach(), d.grad, b.grad)
-    ex = rel_l2(grads[torch.float32][0], grads[torch.float64][0])
-    eg = rel_l2(grads[torch.float32][1], grads[torch.float64][1])
-    eb = abs(float(grads[torch.float32][2]) - float(grads[torch.float64][2])) / abs(float(grads[torch.float64][2]))
-    print(f"unrolled BSREM float32 vs float64: x_K {ex:.2e}, d/dg {eg:.2e}, d/dbeta {eb:.2e}")
-    assert ex <= 1e-4 and eg <= 1e-3 and eb <= 1e-3
+        grads[dt] = (xk.detach(), d.grad, x0.grad, b.grad)
+    r32, r64 = grads[torch.float32], grads[torch.float64]
+    ex, eg, ef = (rel_l2(r32[k], r64[k]) for k in range(3))
+    eb = abs(float(r32[3]) - float(r64[3])) / abs(float(r64[3]))
+    print(f"unrolled BSREM {n}^3 float32 vs float64: x_K {ex:.2e}, d/dg {eg:.2e}, d/dx0 {ef:.2e}, "
+          f"d/dbeta {eb:.2e}")
+    assert ex <= 1e-4 and eg <= bound and ef <= bound and eb <= bound

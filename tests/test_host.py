@@ -63,6 +63,11 @@ def test_c_abi_rejects_bad_arguments_without_touching_the_gpu():
     assert lib.spect_dvf_warp(0, fake, fake2, fake, 1, 4, 4, None) == -1
     assert lib.spect_rdp_hessian_jvp(0, fake, fake2, fake, 0, 4, 4, 1, 1, 1, 0.1, 2, 1e-11, 1, None) == -1
     assert lib.spect_ew_scatter(7, fake, None, fake2, None, 10, 1.0, 1.0, 1.0, 0.5, 0.5, None) == -1
+    # the rotated-frame entry points check the plan before anything else
+    for name in ("spect_rotate_object", "spect_att_volume", "spect_psf_volume"):
+        assert getattr(lib, name)(None, 0, fake, fake2, 1, fake, None) == -1
+        assert b"null plan" in lib.spect_last_error()
+    assert lib.spect_rotate_back(None, 0, fake, fake2, 1, 0, fake, None) == -1
 
 
 def test_no_cpu_fallback():
@@ -76,7 +81,9 @@ def test_no_cpu_fallback():
     pytomography_amd.set_device("cuda" if torch.cuda.is_available() else "cpu")
 
 
-def test_unsupported_transform_orders_raise():
+def test_transform_orders_and_shapes():
+    """[PSF, attenuation] runs the per-angle path (fused is False); a map of
+    the wrong shape is refused."""
     from pytomography_amd.metadata.SPECT import SPECTObjectMeta, SPECTProjMeta, SPECTPSFMeta
     from pytomography_amd.transforms.SPECT import SPECTAttenuationTransform, SPECTPSFTransform
     from pytomography_amd.projectors.SPECT import SPECTSystemMatrix
@@ -84,9 +91,8 @@ def test_unsupported_transform_orders_raise():
     pm = SPECTProjMeta((16, 8), (0.48, 0.48), [0.0, 90.0], radii=[25.0, 25.0])
     att = SPECTAttenuationTransform(torch.zeros((16, 16, 8)))
     psf = SPECTPSFTransform(SPECTPSFMeta((0.03, 0.1)))
-    with pytest.raises(NotImplementedError):
-        SPECTSystemMatrix([psf, att], [], om, pm)
-    SPECTSystemMatrix([att, psf], [], om, pm)
+    assert not SPECTSystemMatrix([psf, att], [], om, pm).fused
+    assert SPECTSystemMatrix([att, psf], [], om, pm).fused
     with pytest.raises(ValueError):
         SPECTSystemMatrix([SPECTAttenuationTransform(torch.zeros((8, 8, 8)))], [], om, pm)
 
@@ -160,3 +166,55 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=300, cwd=root)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_transform_classification_and_proj2proj_signatures():
+    """Fused path only for [attenuation?, PSF?] in that order without
+    proj2proj; a CutOffTransform's forward takes the call's angle indices."""
+    import pytomography_amd
+    from pytomography_amd.projectors.SPECT.spect_system_matrix import _accepts_angles
+    from pytomography_amd.transforms import Transform
+    from pytomography_amd.transforms.SPECT import CutOffTransform, SPECTPSFTransform
+    from tests.helpers import problem_arrays, product_system_matrix
+
+    class Ident(Transform):
+        def forward(self, x, a):
+            return x
+
+        def backward(self, x, a):
+            return x
+
+    class ProjOnly:
+        def forward(self, proj):
+            return proj
+
+    pa = problem_arrays(12, 4, seed=1, att=True, psf=True)
+    try:
+        assert product_system_matrix(pa, device="cpu").fused
+        assert product_system_matrix(pa, device="cpu", obj2obj=["att"]).fused
+        assert not product_system_matrix(pa, device="cpu", obj2obj=["psf", "att"]).fused
+        assert not product_system_matrix(pa, device="cpu", obj2obj=["att", Ident(), "psf"]).fused
+        mask = torch.ones((4, 12, 12))
+        assert not product_system_matrix(pa, device="cpu", proj2proj=[CutOffTransform(mask)]).fused
+    finally:
+        pytomography_amd.set_device("cuda" if torch.cuda.is_available() else "cpu")
+    assert _accepts_angles(CutOffTransform(torch.ones((4, 12, 12))))
+    assert not _accepts_angles(ProjOnly())
+    with pytest.raises(ValueError):
+        SPECTPSFTransform()
+
+
+def test_cutoff_transform_pads_and_follows_subset_angles():
+    """cutoff.py:5-20: the mask is zero-padded along Lr like the projections;
+    for a subset's projections the rows of its angles are used."""
+    from pytomography_amd.transforms.SPECT import CutOffTransform
+    from pytomography_amd.utils import compute_pad_size
+    mask = torch.rand((6, 10, 5), generator=torch.Generator().manual_seed(2))
+    t = CutOffTransform(mask)
+    p = compute_pad_size(10)
+    assert t.mask.shape == (6, 10 + 2 * p, 5)
+    assert float(t.mask[:, :p].abs().max()) == 0.0
+    proj = torch.ones((6, 10 + 2 * p, 5))
+    assert torch.equal(t.forward(proj), t.mask)
+    sub = torch.tensor([1, 4])
+    assert torch.equal(t.backward(proj[:2], sub), t.mask[sub])

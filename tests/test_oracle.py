@@ -613,3 +613,95 @@ def test_dvf_warp_closed_forms(default_float64):
     out = so.dvf_warp(ramp, fr)
     exp = ramp + 0.3 * 0.25 - 0.7 * 0.4 + 1.1 * -0.35
     assert torch.allclose(out[:5, :6, 1:], exp[:5, :6, 1:], rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------ transform lists (per-angle path)
+class _Scale:
+    """A caller's transform: multiply by a constant (backward the same)."""
+
+    def __init__(self, c):
+        self.c = c
+
+    def forward(self, obj_i, ang_idx):
+        return obj_i * self.c
+
+    backward = forward
+
+
+class _MaskAngles:
+    """proj2proj: zero the projections of some angles (a diagonal mask)."""
+
+    def __init__(self, n_angles, zero):
+        self.m = torch.ones(n_angles)
+        self.m[list(zero)] = 0
+
+    def forward(self, proj, angle_indices):
+        return proj * self.m[torch.as_tensor(angle_indices).long()].reshape(-1, 1, 1)
+
+    backward = forward
+
+
+def _small_geometry(att=True, psf=True):
+    n, a = 12, 6
+    f = torch.zeros((n, n, n))
+    f[4:8, 3:9, 2:10] = 1.0
+    mu = torch.full((n, n, n), 0.12) if att else None
+    return so.make_geometry((n, n, n), (0.4, 0.4, 0.4), np.arange(a) * 60.0, radii=np.full(a, 20.0),
+                            mu=mu, psf=so.PSFModel((0.03, 0.1)) if psf else None), f
+
+
+def test_transform_lists_reduce_to_the_default_chain():
+    """The default obj2obj list is [attenuation, PSF]; an identity inserted
+    anywhere changes nothing; a constant factor c scales forward and
+    backward by c (linearity) wherever it sits."""
+    g, f = _small_geometry()
+    y = torch.rand((6, 12, 12), generator=torch.Generator().manual_seed(3))
+    fw, bw = so.forward(g, f), so.backward(g, y)
+    assert torch.equal(so.forward(g, f, obj2obj=["att", "psf"]), fw)
+    assert torch.equal(so.backward(g, y, obj2obj=["att", _Scale(1.0), "psf"]), bw)
+    for pos in (0, 1, 2):
+        ts = ["att", "psf"]
+        ts.insert(pos, _Scale(3.0))
+        assert torch.allclose(so.forward(g, f, obj2obj=ts), 3 * fw, rtol=1e-6, atol=1e-6)
+        assert torch.allclose(so.backward(g, y, obj2obj=ts), 3 * bw, rtol=1e-6, atol=1e-6)
+    # order matters once the transforms do not commute: PSF then attenuation
+    assert not torch.allclose(so.forward(g, f, obj2obj=["psf", "att"]), fw, rtol=1e-4)
+
+
+def test_proj2proj_masks_forward_and_backward():
+    """A diagonal proj2proj mask zeroes exactly its angles' projections in
+    forward, and backward equals the default back projection of the masked
+    projections."""
+    g, f = _small_geometry()
+    m = _MaskAngles(6, zero=(1, 4))
+    out = so.forward(g, f, proj2proj=[m])
+    ref = so.forward(g, f)
+    assert float(out[[1, 4]].abs().max()) == 0.0
+    assert torch.equal(out[[0, 2, 3, 5]], ref[[0, 2, 3, 5]])
+    y = torch.rand((6, 12, 12), generator=torch.Generator().manual_seed(4))
+    ym = y.clone()
+    ym[[1, 4]] = 0
+    assert torch.allclose(so.backward(g, y, proj2proj=[m]), so.backward(g, ym), rtol=1e-6, atol=1e-7)
+    # a subset: the mask rows follow the call's angle indices
+    sub = torch.tensor([1, 3])
+    assert float(so.forward(g, f, sub, proj2proj=[m])[0].abs().max()) == 0.0
+
+
+def test_average_prob_matrix_without_attenuation_is_one():
+    """mu = 0: every probability of detection is 1, so the average over
+    angles is 1 wherever all bilinear taps of the back rotation fall inside
+    the padded frame (the inscribed disk of the object)."""
+    n = 16
+    g = so.make_geometry((n, n, 4), (0.4,) * 3, np.arange(8) * 45.0, mu=torch.zeros((n, n, 4)))
+    avg = so.average_prob_matrix(g)
+    yy, xx = np.meshgrid(np.arange(n) - (n - 1) / 2, np.arange(n) - (n - 1) / 2, indexing="ij")
+    disk = torch.from_numpy(np.hypot(xx, yy) < n / 2 - 1)
+    assert float((avg[disk] - 1).abs().max()) < 1e-6
+
+
+def test_square_taps_closed_form():
+    """get_1D_PSF_layer_square (psf.py:85-101): max(0, 1 - |x|/sigma),
+    normalised; sigma = 2 gives [0, .5, 1, .5, 0] / 2."""
+    t = so.square_taps(np.array([2.0, 1.0]), 5)
+    assert torch.allclose(t[0], torch.tensor([0.0, 0.25, 0.5, 0.25, 0.0]))
+    assert torch.allclose(t[1], torch.tensor([0.0, 0.0, 1.0, 0.0, 0.0]))
