@@ -2,11 +2,14 @@
 
 Drop-in for ``projectors/SPECT/spect_system_matrix.py:16-201``: same
 constructor, ``forward(object, subset_idx)``, ``backward(proj, subset_idx)``,
-subset bookkeeping and sensitivity image.  Supported obj2obj transforms are
+subset bookkeeping and sensitivity image.  With obj2obj transforms
 ``SPECTAttenuationTransform`` followed by ``SPECTPSFTransform`` (either may be
-absent), which is the order the reference's examples and tests use
-(``tests/test_multi_bed_recon.py:68-72``).  Each call is a handful of
-stream-ordered kernel launches (no per-angle Python loop, no host syncs).
+absent; the order the reference's examples and tests use,
+``tests/test_multi_bed_recon.py:68-72``) and no proj2proj transform, each call
+is a handful of stream-ordered fused kernel launches (no per-angle Python
+loop, no host syncs).  Any other transforms run the reference's per-angle loop
+over whole rotated frames, with the rotation, attenuation and PSF on HIP
+kernels (``fused`` is False).
 
 Extras beyond the reference:
   * ``backward_adjoint(proj, subset_idx)`` -- the exact transpose of
