@@ -180,32 +180,46 @@ def bench_batch(args, device):
             out.append(PoissonLogLikelihood(sm, g, additive_term=s))
         return out
 
-    # one-at-a-time GPU loop (same kernels, one problem after another)
+    # one-at-a-time GPU loop (same kernels, one problem after another), each
+    # problem timed on its own
     lks = likelihoods()
     for lk in lks:
         lk._set_n_subsets(subsets)
+    per_problem_s = []
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
     for lk in lks:
         alg = OSEM(lk)
         alg._set_n_subsets = lambda k: None
+        t0 = time.perf_counter()
         for _ in range(iters):
             for k in range(subsets):
                 alg._fused_subiteration(0, k)
-    torch.cuda.synchronize()
-    single_s = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        per_problem_s.append(time.perf_counter() - t0)
+    single_s = sum(per_problem_s)
     del lks
 
     batch = BatchedOSEM(likelihoods(), subsets, use_graph=True)
     batch.run(iters)                      # capture + first replay (warm-up)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    reps = args.steps
+    reps = max(3, args.steps)
+    runs_s = []
     for _ in range(reps):
+        t0 = time.perf_counter()
         batch.run(iters)
-    torch.cuda.synchronize()
-    batch_s = (time.perf_counter() - t0) / reps
+        torch.cuda.synchronize()
+        runs_s.append(time.perf_counter() - t0)
+    batch_s = float(np.median(runs_s))
     n_prob = len(specs)
+    # Ceiling: the work of a problem is N^3 x A per iteration; the best rate
+    # any problem reaches alone (the large ones fill the GPU) prices the whole
+    # mix, so single_s / ideal_s bounds what any batching can gain.
+    work = [n ** 3 * na for n, na, _ in specs]
+    rates = [w / t for w, t in zip(work, per_problem_s)]
+    ideal_s = sum(work) / max(rates)
+    by_size = {}
+    for (n, na, _), r in zip(specs, rates):
+        by_size.setdefault(f"{n}x{na}", []).append(r / max(rates))
     out = {
         "metric": METRIC,
         "value": n_prob * iters / batch_s,
@@ -221,7 +235,13 @@ def bench_batch(args, device):
             "autotune_s": tune_s,
             "one_at_a_time_s": single_s,
             "batched_graph_s": batch_s,
+            "batched_graph_runs_s": runs_s,
             "speedup_vs_one_at_a_time": single_s / batch_s,
+            "work_weighted_ceiling_speedup": single_s / ideal_s,
+            "solo_rate_vs_best": {k: float(np.mean(v)) for k, v in sorted(by_size.items())},
+            "schedule": "per sub-iteration one forward, reduce, field and gather launch per group "
+                        "of problems with equal PSF width and z extent; groups are parallel "
+                        "branches of one hipGraph replayed per run",
         },
     }
     print(json.dumps(out))

@@ -243,7 +243,10 @@ typedef struct spect_batch_problem {
 int spect_batch_create(const spect_batch_problem *probs, int n, int dtype, double delta,
                        double floor_value, spect_batch **out);
 /* n_iters full iterations (all subsets, all problems).  use_graph != 0 captures
- * the schedule into a hipGraph on the first call (per n_iters) and replays it. */
+ * the schedule into a hipGraph on the first call (per n_iters) and replays it;
+ * the groups (problems of one PSF width and z extent) are parallel branches of
+ * that graph, so one group's launches overlap another's.  use_graph == 0 runs
+ * the groups one after another on `stream`. */
 int spect_batch_run(spect_batch *b, int n_iters, int use_graph, void *stream);
 int spect_batch_destroy(spect_batch *b);
 

@@ -374,6 +374,11 @@ struct spect_batch {
   void *tables = nullptr;  // device: job arrays and block starts
   std::vector<std::pair<int, hipGraphExec_t>> graphs;  // (n_iters, exec)
   hipStream_t capture_stream = nullptr;  // private stream the schedule is captured on
+  // one stream and one join event per group: in a captured schedule the
+  // groups are parallel branches (their problems share nothing)
+  std::vector<hipStream_t> group_streams;
+  std::vector<hipEvent_t> group_events;
+  hipEvent_t fork_event = nullptr;
 };
 
 namespace {
@@ -474,32 +479,65 @@ int batch_build(spect_batch *b, const spect_batch_problem *probs, int n, double 
   return SPECT_OK;
 }
 
+// Every sub-iteration of one group's problems: forward, reduce (Poisson
+// ratio), back-projection field, gather with the OSEM update -- one launch
+// each for all members.
 template <typename R>
-int batch_launch(spect_batch *b, int n_iters, hipStream_t st) {
+int batch_launch_group(spect_batch *b, const spect_batch::Group &G, int n_iters, hipStream_t st) {
   auto *base = reinterpret_cast<unsigned char *>(b->tables);
   for (int it = 0; it < n_iters; ++it) {
     for (int sub = 0; sub < b->n_subsets; ++sub) {
-      for (const auto &G : b->groups) {
-        HIP_TRY(launch_forward_batch<R>(reinterpret_cast<const FwdJob<R> *>(base + G.fwd_off[sub]),
-                                        reinterpret_cast<const int *>(base + G.starts_off[0][sub]),
-                                        G.nprob, G.blocks[0][sub], G.H, G.Nz, G.maxL, G.af, st));
-      }
-      for (const auto &G : b->groups) {
-        HIP_TRY(launch_reduce_batch<R>(reinterpret_cast<const ReduceJob<R> *>(base + G.red_off[sub]),
-                                       reinterpret_cast<const int *>(base + G.starts_off[1][sub]),
-                                       G.nprob, G.blocks[1][sub], st));
-      }
-      for (const auto &G : b->groups) {
-        HIP_TRY(launch_field_batch<R>(reinterpret_cast<const FieldJob<R> *>(base + G.fld_off[sub]),
-                                      reinterpret_cast<const int *>(base + G.starts_off[2][sub]),
-                                      G.nprob, G.blocks[2][sub], G.H, G.Nz, G.maxL, G.af, st));
-      }
-      for (const auto &G : b->groups) {
-        HIP_TRY(launch_gather_batch<R>(reinterpret_cast<const GatherJob<R> *>(base + G.gat_off[sub]),
-                                       reinterpret_cast<const int *>(base + G.starts_off[3][sub]),
-                                       G.nprob, G.blocks[3][sub], SPECT_BP_REFERENCE, st));
-      }
+      HIP_TRY(launch_forward_batch<R>(reinterpret_cast<const FwdJob<R> *>(base + G.fwd_off[sub]),
+                                      reinterpret_cast<const int *>(base + G.starts_off[0][sub]),
+                                      G.nprob, G.blocks[0][sub], G.H, G.Nz, G.maxL, G.af, st));
+      HIP_TRY(launch_reduce_batch<R>(reinterpret_cast<const ReduceJob<R> *>(base + G.red_off[sub]),
+                                     reinterpret_cast<const int *>(base + G.starts_off[1][sub]),
+                                     G.nprob, G.blocks[1][sub], st));
+      HIP_TRY(launch_field_batch<R>(reinterpret_cast<const FieldJob<R> *>(base + G.fld_off[sub]),
+                                    reinterpret_cast<const int *>(base + G.starts_off[2][sub]),
+                                    G.nprob, G.blocks[2][sub], G.H, G.Nz, G.maxL, G.af, st));
+      HIP_TRY(launch_gather_batch<R>(reinterpret_cast<const GatherJob<R> *>(base + G.gat_off[sub]),
+                                     reinterpret_cast<const int *>(base + G.starts_off[3][sub]),
+                                     G.nprob, G.blocks[3][sub], SPECT_BP_REFERENCE, st));
     }
+  }
+  return SPECT_OK;
+}
+
+// The whole schedule on one stream, group after group.
+template <typename R>
+int batch_launch(spect_batch *b, int n_iters, hipStream_t st) {
+  for (const auto &G : b->groups)
+    if (int rc = batch_launch_group<R>(b, G, n_iters, st)) return rc;
+  return SPECT_OK;
+}
+
+// A stream and a join event per group, and the fork event (created before
+// any capture begins).
+int batch_branch_streams(spect_batch *b) {
+  while (b->group_streams.size() < b->groups.size()) {
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    b->group_streams.push_back(s);
+    hipEvent_t e = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->group_events.push_back(e);
+  }
+  if (!b->fork_event) HIP_TRY(hipEventCreateWithFlags(&b->fork_event, hipEventDisableTiming));
+  return SPECT_OK;
+}
+
+// The schedule with every group on its own stream, forked from and joined
+// back into st (under capture: parallel branches of the graph).
+template <typename R>
+int batch_launch_branches(spect_batch *b, int n_iters, hipStream_t st) {
+  const size_t ng = b->groups.size();
+  HIP_TRY(hipEventRecord(b->fork_event, st));
+  for (size_t k = 0; k < ng; ++k) {
+    HIP_TRY(hipStreamWaitEvent(b->group_streams[k], b->fork_event, 0));
+    if (int rc = batch_launch_group<R>(b, b->groups[k], n_iters, b->group_streams[k])) return rc;
+    HIP_TRY(hipEventRecord(b->group_events[k], b->group_streams[k]));
+    HIP_TRY(hipStreamWaitEvent(st, b->group_events[k], 0));
   }
   return SPECT_OK;
 }
@@ -579,10 +617,11 @@ int spect_batch_run(spect_batch *b, int n_iters, int use_graph, void *stream) {
   // Capture on a private stream (the caller's may be the null stream, which
   // cannot be captured); the graph is then launched on the caller's stream.
   if (!b->capture_stream) HIP_TRY(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+  if (int rc = batch_branch_streams(b)) return rc;
   hipStream_t cs = b->capture_stream;
   HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-  const int rc = b->dtype == SPECT_F32 ? batch_launch<float>(b, n_iters, cs)
-                                       : batch_launch<double>(b, n_iters, cs);
+  const int rc = b->dtype == SPECT_F32 ? batch_launch_branches<float>(b, n_iters, cs)
+                                       : batch_launch_branches<double>(b, n_iters, cs);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(cs, &graph);
   if (rc) {
@@ -603,6 +642,9 @@ int spect_batch_destroy(spect_batch *b) {
   if (!b) return SPECT_OK;
   for (auto &e : b->graphs) (void)hipGraphExecDestroy(e.second);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  for (hipStream_t s : b->group_streams) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b->group_events) (void)hipEventDestroy(e);
+  if (b->fork_event) (void)hipEventDestroy(b->fork_event);
   (void)hipFree(b->tables);
   delete b;
   return SPECT_OK;
