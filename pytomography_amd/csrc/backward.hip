@@ -52,7 +52,7 @@ __global__ __launch_bounds__(1024) void field_batch_kernel(const FieldJob<R> *__
 template <typename R, int RZ, int QS, int H, bool AF>
 __global__ __launch_bounds__(1024) void field_sched_kernel(FieldJob<R> J, int nblocks) {
   extern __shared__ __align__(16) unsigned char smem[];
-  field_sched_body<R, RZ, QS, H, AF>(J, nblocks, blockIdx.x, smem);
+  field_sched_body<R, RZ, QS, H, AF>(J, nblocks, xcd_piece(blockIdx.x, gridDim.x), smem);
 }
 
 // In place: Dsum[a][c] <- sum_{c' <= c} Dsum[a][c'] (prefix over chunks, FP64).

@@ -24,11 +24,25 @@ constexpr int kTJ = 32;  // rotated-frame columns per tile (forward and back pro
 // Most planes in one piece of the forward's balanced schedule.
 constexpr int kPieceMax = 32;
 
-// Blocks the scheduled forward launches for n angles: the resident blocks,
-// or more when even kPieceMax-plane pieces outnumber them (large calls).
+// XCDs of the device; blocks are dispatched to them round-robin (block b
+// runs on XCD b % kXcds).
+constexpr int kXcds = 8;
+
+// Blocks a scheduled launch runs for n angles: the resident blocks, or more
+// when even kPieceMax-plane pieces outnumber them (large calls), rounded up to
+// a multiple of kXcds.
 __host__ __device__ inline int sched_grid_blocks(int n, int ntiles, int P, int resident) {
   const int most = n * ntiles * ((P + kPieceMax - 1) / kPieceMax);
-  return most > resident ? most : resident;
+  const int g = most > resident ? most : resident;
+  return (g + kXcds - 1) / kXcds * kXcds;
+}
+
+// The piece block b of a scheduled launch of `grid` blocks (a multiple of
+// kXcds) takes: each XCD works through one contiguous run of pieces -- whole
+// (angle, tile) pairs, whose object and attenuation samples then stay in that
+// XCD's L2 instead of being fetched by all eight.  A permutation of [0, grid).
+__host__ __device__ inline int xcd_piece(int b, int grid) {
+  return (b % kXcds) * (grid / kXcds) + b / kXcds;
 }
 
 template <typename R> struct V2;
@@ -393,7 +407,7 @@ struct SchedLoc {
 
 // The schedule for nblocks resident blocks, evaluated by the whole block
 // (blockDim.x a multiple of 64, at most 1024; every thread calls), located at
-// piece b.  With `out`, block b = 0 writes the schedule there.
+// piece b.  With `out`, the block of piece 0 writes the schedule there.
 template <typename R>
 __device__ __forceinline__ SchedLoc sched_locate(const Geo<R> &g, const int32_t *__restrict__ angle_idx,
                                                  const int2 *__restrict__ ranges, int n, int nblocks,

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(512) void fwd_kernel(FwdJob<R> J) {
 template <typename R, int RZ, int QS, int H, bool AF>
 __global__ __launch_bounds__(512) void fwd_sched_kernel(FwdJob<R> J, int *__restrict__ sched, int nblocks) {
   extern __shared__ __align__(16) unsigned char smem[];
-  fwd_sched_body<R, RZ, QS, H, AF>(J, J.tb.ranges, sched, nblocks, blockIdx.x, smem);
+  fwd_sched_body<R, RZ, QS, H, AF>(J, J.tb.ranges, sched, nblocks, xcd_piece(blockIdx.x, gridDim.x), smem);
 }
 
 template <typename R>

@@ -619,8 +619,8 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
 // sched layout (ints): [0] total pieces, [1] Q, then base[0..npairs]
 // (base[p] = pieces of the pairs before p).
 
-// Block b of the scheduled forward: piece b, written to partial slot b; block
-// 0 writes the schedule to `sched` for the reduce.
+// The block of the scheduled forward that takes piece b: it writes partial
+// slot b; piece 0's block writes the schedule to `sched` for the reduce.
 template <typename R, int RZ, int QS, int H, bool AF>
 __device__ __forceinline__ void fwd_sched_body(const FwdJob<R> &J, const int2 *__restrict__ ranges,
                                                int *__restrict__ sched, int nblocks, int b,

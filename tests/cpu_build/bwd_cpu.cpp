@@ -151,8 +151,9 @@ int main(int argc, char **argv) {
     constexpr int HH = decltype(h_c)::value;
     if (sched_mode) {
       cpu_build::run_grid(grid, 1, 1, bt, smem, [&](int b, int, int, unsigned char *s) {
-        if (att.attf) field_sched_body<float, RZ, QS, HH, true>(FJ, sched_blocks, b, s);
-        else field_sched_body<float, RZ, QS, HH, false>(FJ, sched_blocks, b, s);
+        const int piece = xcd_piece(b, grid);  // field_sched_kernel's block -> piece order
+        if (att.attf) field_sched_body<float, RZ, QS, HH, true>(FJ, sched_blocks, piece, s);
+        else field_sched_body<float, RZ, QS, HH, false>(FJ, sched_blocks, piece, s);
       });
       return;
     }

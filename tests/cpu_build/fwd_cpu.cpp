@@ -56,9 +56,10 @@ int main(int argc, char **argv) {
     constexpr int HH = decltype(h_c)::value;
     if (sched_mode) {
       cpu_build::run_grid(grid, 1, 1, bt, smem, [&](int b, int, int, unsigned char *s) {
-        if (att.attf) fwd_sched_body<float, RZ, QS, HH, true>(J, p.tables().ranges, sched.data(), sched_blocks, b, s);
-        else fwd_sched_body<float, RZ, QS, HH, false>(J, p.tables().ranges, sched.data(), sched_blocks, b,
-                                                        s);
+        const int piece = xcd_piece(b, grid);  // fwd_sched_kernel's block -> piece order
+        if (att.attf) fwd_sched_body<float, RZ, QS, HH, true>(J, p.tables().ranges, sched.data(), sched_blocks, piece, s);
+        else fwd_sched_body<float, RZ, QS, HH, false>(J, p.tables().ranges, sched.data(), sched_blocks,
+                                                        piece, s);
       });
       return;
     }
