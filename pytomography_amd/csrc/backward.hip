@@ -232,7 +232,7 @@ hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
                             const R *nrm, R delta, R floor_value, const double *Wm,
                             const double *Pref, R *out_mu, hipStream_t st, size_t w_astride,
                             size_t w_rstride) {
-  constexpr int ZV = sizeof(R) == 4 ? 4 : 2;
+  constexpr int ZV = gather_zv<R>();
   const unsigned blocks = (unsigned)gather_blocks(g);
   const GatherJob<R> J{g,  tb,      W,      idx,     n,          out_mode, out, f, nrm,
                        delta, floor_value, Wm, Pref, out_mu, w_astride, w_rstride};
@@ -280,7 +280,7 @@ hipError_t launch_field_batch(const FieldJob<R> *jobs, const int *starts, int nj
 template <typename R>
 hipError_t launch_gather_batch(const GatherJob<R> *jobs, const int *starts, int njobs, int blocks,
                                int mode, hipStream_t st) {
-  constexpr int ZV = sizeof(R) == 4 ? 4 : 2;
+  constexpr int ZV = gather_zv<R>();
   if (mode == SPECT_BP_REFERENCE)
     gather_batch_kernel<R, ZV, 0><<<blocks, 256, 0, st>>>(jobs, starts, njobs);
   else

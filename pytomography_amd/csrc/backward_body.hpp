@@ -104,7 +104,11 @@ __device__ __forceinline__ void field_body_range(const FieldJob<R> &J, int al, i
       src[buf][tid] = make_src<R>(ix, iy, g.pad);
       inside = in_footprint<R>(ix, iy, g);
     }
-    if (!__syncthreads_or(inside)) continue;
+    // A scheduled piece (c < 0) covers planes of the activity table's range
+    // only, so the plane is taken without the block-wide test (a plane of
+    // that superset with no sample inside gets W written all the same).
+    if (c < 0) __syncthreads();
+    else if (!__syncthreads_or(inside)) continue;
     R bz[RZ];
     // AF: this plane's attenuation factors are requested before the z blur
     // (after the y-blur barrier, which would otherwise wait for them) so that
@@ -278,15 +282,22 @@ __device__ __forceinline__ void bp_store(int out_mode, R *__restrict__ out, size
 // Vector of ZV consecutive z values.
 template <typename R, int ZV> struct ZVec { R v[ZV]; };
 
+// 16-byte vector accesses when ZV values fill whole 16-byte words.
 template <typename R, int ZV>
 __device__ __forceinline__ ZVec<R, ZV> load_zvec(const R *p) {
   ZVec<R, ZV> r;
-  if constexpr (sizeof(R) == 4 && ZV == 4) {
-    const float4 x = *reinterpret_cast<const float4 *>(p);
-    r.v[0] = x.x; r.v[1] = x.y; r.v[2] = x.z; r.v[3] = x.w;
-  } else if constexpr (sizeof(R) == 8 && ZV == 2) {
-    const double2 x = *reinterpret_cast<const double2 *>(p);
-    r.v[0] = x.x; r.v[1] = x.y;
+  if constexpr (sizeof(R) == 4 && ZV % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < ZV; q += 4) {
+      const float4 x = *reinterpret_cast<const float4 *>(p + q);
+      r.v[q] = x.x; r.v[q + 1] = x.y; r.v[q + 2] = x.z; r.v[q + 3] = x.w;
+    }
+  } else if constexpr (sizeof(R) == 8 && ZV % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < ZV; q += 2) {
+      const double2 x = *reinterpret_cast<const double2 *>(p + q);
+      r.v[q] = x.x; r.v[q + 1] = x.y;
+    }
   } else {
 #pragma unroll
     for (int m = 0; m < ZV; ++m) r.v[m] = p[m];
@@ -296,10 +307,13 @@ __device__ __forceinline__ ZVec<R, ZV> load_zvec(const R *p) {
 
 template <typename R, int ZV>
 __device__ __forceinline__ void store_zvec(R *p, const R (&v)[ZV]) {
-  if constexpr (sizeof(R) == 4 && ZV == 4) {
-    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
-  } else if constexpr (sizeof(R) == 8 && ZV == 2) {
-    *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+  if constexpr (sizeof(R) == 4 && ZV % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < ZV; q += 4)
+      *reinterpret_cast<float4 *>(p + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
+  } else if constexpr (sizeof(R) == 8 && ZV % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < ZV; q += 2) *reinterpret_cast<double2 *>(p + q) = make_double2(v[q], v[q + 1]);
   } else {
 #pragma unroll
     for (int m = 0; m < ZV; ++m) p[m] = v[m];

@@ -82,8 +82,12 @@ template <typename R> int fwd_blocks(const Geo<R> &g, int n) { return n * g.ntil
 template <typename R> int reduce_blocks(const Geo<R> &g, int n) {
   return (int)(((size_t)n * (g.jo1 - g.jo0) * g.Nz + 255) / 256);
 }
+// z values per lane of the gather kernels: 32 bytes (two 16-byte loads per
+// bilinear tap), so each angle's coordinate and tap arithmetic is shared by
+// eight float (four double) outputs.
+template <typename R> constexpr int gather_zv() { return 32 / (int)sizeof(R); }
 template <typename R> int gather_blocks(const Geo<R> &g) {
-  constexpr int ZV = sizeof(R) == 4 ? 4 : 2;
+  constexpr int ZV = gather_zv<R>();
   return (int)(((size_t)g.N * g.N * ((g.Nz + ZV - 1) / ZV) + 255) / 256);
 }
 

@@ -83,7 +83,7 @@ int run_grad(const std::string &dir, const cpu_build::Problem &p, const std::vec
         v = acc;
       }
     }
-  constexpr int ZV = 4;
+  constexpr int ZV = gather_zv<float>();  // as launch_bp_gather
   std::vector<float> grad_f((size_t)N * N * Nz, std::numeric_limits<float>::quiet_NaN());
   std::vector<float> grad_mu(grad_f.size(), std::numeric_limits<float>::quiet_NaN());
   const GatherJob<float> GJ{g, tb, W.data(), p.sel.data(), n, SPECT_OUT_STORE, grad_f.data(), nullptr,
@@ -172,8 +172,8 @@ int main(int argc, char **argv) {
   }
   const size_t w_written = W.size() - cpu_build::count_nan(W);
 
-  // launch_bp_gather: one lane per (x, y, 4 z); no barriers
-  constexpr int ZV = 4;
+  // launch_bp_gather: one lane per (x, y, ZV z); no barriers
+  constexpr int ZV = gather_zv<float>();  // as launch_bp_gather
   const size_t lanes = (size_t)N * N * ((Nz + ZV - 1) / ZV);
   for (int mode = 0; mode < 2; ++mode) {
     std::vector<float> out((size_t)N * N * Nz, std::numeric_limits<float>::quiet_NaN());
