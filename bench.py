@@ -619,7 +619,7 @@ def measure_osem(n: int, na: int, ns: int, att: bool, psf: bool, device, steps: 
     fwd_b, bwd_b, upd_b, ratio_b = algorithmic_bytes(n, a_s, att)
     # Dominant kernel: the forward projection (fwd_kernel + its chunk reduce),
     # timed above with HIP events on the stream it is launched on.
-    kern = f"fwd_kernel + fwd_reduce_kernel (spect_forward, {a_s} angles)"
+    kern = f"fwd_sched_kernel + reduce_sched_kernel (spect_forward, {a_s} angles)"
     achieved = fwd_b / (fwd_ms * 1e-3) / 1e9
     iter_bytes = ns * (fwd_b + bwd_b + upd_b + ratio_b)
 
