@@ -20,7 +20,7 @@ namespace spect {
 // chunk total Dsum (FP64); the gather subtracts the prefix of chunk totals.
 // Planes [i_lo, i_hi) of (angle al, tile jt); c is the chunk whose FP64
 // suffix-table rows start the attenuation march (suffix-table mode only).
-template <typename R, int RZ, int QS, int H, bool GRAD, bool AF>
+template <typename R, int RZ, int QS, int H, bool GRAD, bool AF, bool MARCH = !AF>
 __device__ __forceinline__ void field_body_range(const FieldJob<R> &J, int al, int jt, int c,
                                                  int i_lo, int i_hi, unsigned char *smem) {
   const Geo<R> &g = J.g;
@@ -59,7 +59,9 @@ __device__ __forceinline__ void field_body_range(const FieldJob<R> &J, int al, i
   const bool bact = run < runs;
   const bool mine = bact && jl < tj;
   const auto cs = ld_uniform(tb.trig + ang);
-  const bool has_mu = mu != nullptr;
+  // MARCH: mu may be given and is marched here (FP64 suffix sums); a scheduled
+  // piece runs with the attenuation field or without attenuation, never both
+  const bool has_mu = MARCH && mu != nullptr;
 
   // projection window for the y blur: tile rows [half*16 - H, half*16 + 16 + H)
   R pw[HALF_ROWS + 2 * H];
@@ -71,11 +73,11 @@ __device__ __forceinline__ void field_body_range(const FieldJob<R> &J, int al, i
       if (j >= g.jo0 && j < g.jo1) pw[m] = proj[((size_t)al * nout + (j - g.jo0)) * Nz + za];
     }
   }
-  double S[AF ? 1 : RZ];
+  double S[MARCH ? RZ : 1];
 #pragma unroll
-  for (int m = 0; m < (AF ? 1 : RZ); ++m) {
+  for (int m = 0; m < (MARCH ? RZ : 1); ++m) {
     S[m] = 0.0;
-    if (!AF && has_mu && mine && z0 + m < Nz) {
+    if (MARCH && has_mu && mine && z0 + m < Nz) {
       const int rowa = att_global ? ang : al;
       S[m] = att[(((size_t)rowa * g.C + c) * n_jext + (j0 - g.je0 + jl)) * Nz + z0 + m];
     }
@@ -104,15 +106,9 @@ __device__ __forceinline__ void field_body_range(const FieldJob<R> &J, int al, i
       src[buf][tid] = make_src<R>(ix, iy, g.pad);
       inside = in_footprint<R>(ix, iy, g);
     }
-    // A scheduled piece (c < 0) covers planes of the activity table's range
-    // only, so the plane is taken without the block-wide test (a plane of
-    // that superset with no sample inside gets W written all the same).
-    if (c < 0) __syncthreads();
-    else if (!__syncthreads_or(inside)) continue;
-    R bz[RZ];
-    // AF: this plane's attenuation factors are requested before the z blur
-    // (after the y-blur barrier, which would otherwise wait for them) so that
-    // their HBM latency overlaps the blur arithmetic
+    // AF: this plane's attenuation factors are requested after the y-blur
+    // barrier, so their HBM latency overlaps the z blur (requested before the
+    // y blur as well, they measured 10 us slower per 15-angle launch).
     R a[AF ? RZ : 1];
     auto load_att = [&]() {
       if constexpr (AF) {
@@ -121,6 +117,12 @@ __device__ __forceinline__ void field_body_range(const FieldJob<R> &J, int al, i
                           Nz, a);
       }
     };
+    // A scheduled piece (c < 0) covers planes of the activity table's range
+    // only, so the plane is taken without the block-wide test (a plane of
+    // that superset with no sample inside gets W written all the same).
+    if (c < 0) __syncthreads();
+    else if (!__syncthreads_or(inside)) continue;
+    R bz[RZ];
     if constexpr (H > 0) {
       // ---- A: y blur of the projection with this plane's taps -> U, over
       // the plane's tier of taps [H - HE, H + HE]
@@ -182,14 +184,16 @@ __device__ __forceinline__ void field_body_range(const FieldJob<R> &J, int al, i
       if constexpr (AF) {
 #pragma unroll
         for (int m = 0; m < RZ; ++m) bz[m] = bz[m] * a[m];
-      } else if (has_mu) {
-        R x[RZ];
-        gather_run<R, RZ>(mu, t, z0, Nz, x);
+      } else if constexpr (MARCH) {
+        if (has_mu) {
+          R x[RZ];
+          gather_run<R, RZ>(mu, t, z0, Nz, x);
 #pragma unroll
-        for (int m = 0; m < RZ; ++m) {
-          const R xm = x[m] * g.dx;
-          S[m] += (double)xm;
-          bz[m] = bz[m] * att_factor<R>(S[m], xm);
+          for (int m = 0; m < RZ; ++m) {
+            const R xm = x[m] * g.dx;
+            S[m] += (double)xm;
+            bz[m] = bz[m] * att_factor<R>(S[m], xm);
+          }
         }
       }
       if constexpr (GRAD) {
@@ -254,7 +258,7 @@ __device__ __forceinline__ void field_sched_body(const FieldJob<R> &J, int nbloc
   const int2 r = J.tb.ranges[(size_t)ang * g.ntiles + jt];
   int i_lo, i_hi;
   sched_piece_planes(r, s.k, s.c, i_lo, i_hi);
-  field_body_range<R, RZ, QS, H, false, AF>(J, al, jt, -1, i_lo, i_hi, smem);
+  field_body_range<R, RZ, QS, H, false, AF, false>(J, al, jt, -1, i_lo, i_hi, smem);
 }
 
 // Output stage of a back projection for one voxel value b.
