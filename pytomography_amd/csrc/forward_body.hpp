@@ -170,7 +170,9 @@ __device__ __forceinline__ FwdPiece chunk_piece(const Geo<R> &g, int n, int al, 
   return pc;
 }
 
-template <typename R, int RZ, int QS, int H, bool TRB, bool AF>
+// MARCH: mu may be given without the attenuation field and is marched here
+// (FP64 suffix sums per lane); a scheduled piece never does.
+template <typename R, int RZ, int QS, int H, bool TRB, bool AF, bool MARCH = !AF>
 __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, const FwdPiece &pc,
                                          unsigned char *smem) {
   constexpr int K = 2 * H + 1;
@@ -223,7 +225,7 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
   const int half = tid / nzc, zc = tid - half * nzc;
   const bool cact = half < QS && zc < Nz;
   const auto cs = ld_uniform(tb.trig + ang);
-  const bool has_mu = mu != nullptr;
+  const bool has_mu = MARCH && mu != nullptr;
 
   // The LDS entries the phases read but never write start at zero: V's
   // halo pads, V's z slots past Nz, and whole rows of both buffers past the
@@ -248,14 +250,14 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
         Vz[(size_t)tj * ldv + q] = R(0);
       }
   }
-  double S[AF ? 1 : NPA][ZV];
+  double S[MARCH ? NPA : 1][ZV];
 #pragma unroll
-  for (int p = 0; p < (AF ? 1 : NPA); ++p) {
+  for (int p = 0; p < (MARCH ? NPA : 1); ++p) {
     const int jj = aco + p * cpp;
 #pragma unroll
     for (int e = 0; e < ZV; ++e) {
       S[p][e] = 0.0;
-      if (!AF && has_mu && aact && jj < tj && za + e < Nz) {
+      if (MARCH && has_mu && aact && jj < tj && za + e < Nz) {
         const int rowa = att_global ? ang : al;
         S[p][e] = att[(((size_t)rowa * g.C + c) * n_jext + (j0 - g.je0 + jj)) * Nz + za + e];
       }
@@ -264,6 +266,15 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
   R acc[JH];
 #pragma unroll
   for (int q = 0; q < JH; ++q) acc[q] = R(0);
+  // RAY (no PSF, forward gather): the ray sums are accumulated by the
+  // gathering lanes themselves, in registers, in the same plane order -- no
+  // LDS round trip and one barrier per plane (for the taps' double buffer)
+  constexpr bool RAY = H == 0 && !TRB;
+  R accA[RAY ? NPA : 1][ZV];
+#pragma unroll
+  for (int p = 0; p < (RAY ? NPA : 1); ++p)
+#pragma unroll
+    for (int e = 0; e < ZV; ++e) accA[p][e] = R(0);
 
   const int rix = ld_uniform(tb.radius_index + ang);
   const int i_lo = pc.i_lo, i_hi = pc.i_hi;
@@ -402,7 +413,7 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
         // pass otherwise, to stay within the register budget); columns past the
         // tile load a valid column's taps and are not stored, so no divergent
         // branch separates the loads
-        constexpr int GA = (sizeof(R) == 4 && AF) ? NPA : 1;
+        constexpr int GA = (sizeof(R) == 4 && !MARCH) ? NPA : 1;
 #pragma unroll
         for (int p0 = 0; p0 < NPA; p0 += GA) {
         R raw[GA][4][ZV];
@@ -430,19 +441,26 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
           if constexpr (PF) {
 #pragma unroll
             for (int e = 0; e < ZV; ++e) val[e] = val[e] * Ar[p][e];
-          } else if (has_mu) {
-            R x[ZV];
-            interp_zv<R, ZV>(mu, tt[g], za, Nz, true, x);
+          } else if constexpr (MARCH) {
+            if (has_mu) {
+              R x[ZV];
+              interp_zv<R, ZV>(mu, tt[g], za, Nz, true, x);
 #pragma unroll
-            for (int e = 0; e < ZV; ++e) {
-              const R xm = x[e] * J.g.dx;
-              S[AF ? 0 : p][e] += (double)xm;
-              val[e] = val[e] * att_factor<R>(S[AF ? 0 : p][e], xm);
+              for (int e = 0; e < ZV; ++e) {
+                const R xm = x[e] * J.g.dx;
+                S[AF ? 0 : p][e] += (double)xm;
+                val[e] = val[e] * att_factor<R>(S[AF ? 0 : p][e], xm);
+              }
             }
           }
           if (jj < tj) {
+            if constexpr (RAY) {
 #pragma unroll
-            for (int e = 0; e < ZV; ++e) dstA[(size_t)jj * ldv + e] = val[e];
+              for (int e = 0; e < ZV; ++e) accA[p][e] += val[e];
+            } else {
+#pragma unroll
+              for (int e = 0; e < ZV; ++e) dstA[(size_t)jj * ldv + e] = val[e];
+            }
           }
         }
         }
@@ -488,19 +506,26 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
                            Nz, VEC, a);
 #pragma unroll
             for (int e = 0; e < ZV; ++e) val[e] = val[e] * a[e];
-          } else if (has_mu) {
-            R x[ZV];
-            interp_zv<R, ZV>(mu, t, za, Nz, VEC, x);
+          } else if constexpr (MARCH) {
+            if (has_mu) {
+              R x[ZV];
+              interp_zv<R, ZV>(mu, t, za, Nz, VEC, x);
 #pragma unroll
-            for (int e = 0; e < ZV; ++e) {
-              const R xm = x[e] * g.dx;
-              S[AF ? 0 : p][e] += (double)xm;
-              val[e] = val[e] * att_factor<R>(S[AF ? 0 : p][e], xm);
+              for (int e = 0; e < ZV; ++e) {
+                const R xm = x[e] * g.dx;
+                S[AF ? 0 : p][e] += (double)xm;
+                val[e] = val[e] * att_factor<R>(S[AF ? 0 : p][e], xm);
+              }
             }
           }
+          if constexpr (RAY) {
 #pragma unroll
-          for (int e = 0; e < ZV; ++e)
-            if (VEC || za + e < Nz) dstA[(size_t)jj * ldv + e] = val[e];
+            for (int e = 0; e < ZV; ++e) accA[p][e] += val[e];
+          } else {
+#pragma unroll
+            for (int e = 0; e < ZV; ++e)
+              if (VEC || za + e < Nz) dstA[(size_t)jj * ldv + e] = val[e];
+          }
         }
       }
     };
@@ -509,7 +534,13 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
       else phase_a(std::false_type{});
     }
     if (inext >= 0 && tid < tj) setup(inext, b ^ 1);
-    if constexpr (H == 0) {
+    if constexpr (RAY) {
+      // the next plane's taps are complete; this plane's may be rewritten
+      __syncthreads();
+      if constexpr (PF) {
+        if (inext >= 0) issue(inext, b ^ 1);
+      }
+    } else if constexpr (H == 0) {
       __syncthreads();
       if constexpr (PF) {
         if (inext >= 0) issue(inext, b ^ 1);
@@ -591,7 +622,20 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
     i = inext;
     b ^= 1;
   }
-  if (cact) {
+  if constexpr (RAY) {
+    // every column of the tile (zeros past tj), z < Nz
+    if (aact) {
+      R *dst = partial + pc.slot * JO * Nz;
+#pragma unroll
+      for (int p = 0; p < NPA; ++p) {
+        const int jj = aco + p * cpp;
+        if (jj < kTJ)
+#pragma unroll
+          for (int e = 0; e < ZV; ++e)
+            if (za + e < Nz) dst[(size_t)jj * Nz + za + e] = accA[p][e];
+      }
+    }
+  } else if (cact) {
     R *dst = partial + pc.slot * JO * Nz;
 #pragma unroll
     for (int q = 0; q < JH; ++q) {
@@ -636,7 +680,7 @@ __device__ __forceinline__ void fwd_sched_body(const FwdJob<R> &J, const int2 *_
   pc.c = -1;
   pc.ranged = 1;
   pc.slot = (size_t)b;
-  fwd_body<R, RZ, QS, H, false, AF>(J, al, jt, pc, smem);
+  fwd_body<R, RZ, QS, H, false, AF, false>(J, al, jt, pc, smem);
 }
 
 // Reduce of the scheduled forward: the pieces of the <= 3 tiles a column
