@@ -16,6 +16,8 @@
 // bp_gather_kernel<MODE 0>: W gathered at theta_b = 270 - beta, summed over
 //   the angles in registers, cropped, then stored / added / used for the OSEM
 //   update.
+// bp_gather_kernel<MODE 3>: MODE 0 on a field constant along the rows (the
+//   plain back projection of the projection itself).
 // bp_gather_kernel<MODE 1>: the exact transpose of the forward's bilinear
 //   gather, written as a gather over the <= 16 rotated-frame candidates of
 //   each voxel (race-free and deterministic); MODE 2 adds the mu gradient.
@@ -236,7 +238,9 @@ hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
   const unsigned blocks = (unsigned)gather_blocks(g);
   const GatherJob<R> J{g,  tb,      W,      idx,     n,          out_mode, out, f, nrm,
                        delta, floor_value, Wm, Pref, out_mu, w_astride, w_rstride};
-  if (mode == SPECT_BP_REFERENCE)
+  if (mode == SPECT_BP_REFERENCE && w_astride != 0 && w_rstride == 0)
+    bp_gather_kernel<R, ZV, 3><<<blocks, 256, 0, st>>>(J);  // rows alias one projection row
+  else if (mode == SPECT_BP_REFERENCE)
     bp_gather_kernel<R, ZV, 0><<<blocks, 256, 0, st>>>(J);
   else if (mode == SPECT_BP_ADJOINT)
     bp_gather_kernel<R, ZV, 1><<<blocks, 256, 0, st>>>(J);

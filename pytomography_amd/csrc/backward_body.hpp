@@ -454,6 +454,33 @@ __device__ __forceinline__ void gather_body(const GatherJob<R> &J, size_t gid) {
       }
 #pragma unroll
       for (int m = 0; m < ZV; ++m) acc[m] = acc[m] + v[m];
+    } else if constexpr (MODE == 3) {
+      // MODE 0 on a field whose rows all hold the same values (the plain
+      // back projection reads the projection itself): the two row weights
+      // of a column add up, two loads per sample instead of four
+      R ix, iy;
+      rot_coord<R>(cs.x, -cs.y, linY, linX, g.half, ix, iy);  // theta_b = -theta_f
+      const R fx = floor(ix), fy = floor(iy);
+      const int c0 = int(fx), r0 = int(fy);
+      const R tx = ix - fx, ty = iy - fy;
+      const R ex = R(1) - tx, ny = R(1) - ty;
+      const R wrow = ((unsigned)r0 < (unsigned)g.P ? ny : R(0)) +
+                     ((unsigned)(r0 + 1) < (unsigned)g.P ? ty : R(0));
+      const int cc[2] = {c0, c0 + 1};
+      const R wc[2] = {ex, tx};
+      R v[ZV];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const bool ok = cc[b] >= g.je0 && cc[b] < g.je1;
+        const int cl = min(max(cc[b], g.je0), g.je1 - 1);
+        const R w = ok ? wrow * wc[b] : R(0);
+        R t[ZV];
+        fetch(Wa + (size_t)(cl - g.je0) * Nz + z0, t);
+#pragma unroll
+        for (int m = 0; m < ZV; ++m) v[m] = b == 0 ? t[m] * w : v[m] + t[m] * w;
+      }
+#pragma unroll
+      for (int m = 0; m < ZV; ++m) acc[m] = acc[m] + v[m];
     } else {
       // rotated-frame position whose forward sample lands on (row X, col Y)
       R jc, ic;

@@ -185,6 +185,16 @@ int main(int argc, char **argv) {
     }
     cpu_build::write_file(dir + (mode == 0 ? "/bp_ref.bin" : "/bp_adj.bin"), out);
   }
+  if (H == 0 && p.mu.empty()) {
+    // the direct plain back projection (backward_impl without a field): the
+    // projection itself as a field whose rows all alias one row, gather mode 3
+    std::vector<float> out((size_t)N * N * Nz, std::numeric_limits<float>::quiet_NaN());
+    const size_t nout = (size_t)(g.jo1 - g.jo0);
+    const GatherJob<float> GJ{g, p.tables(), proj.data(), p.sel.data(), n, SPECT_OUT_STORE, out.data(),
+                              nullptr, nullptr, 0.0f, 0.0f, nullptr, nullptr, nullptr, nout * Nz, 0};
+    for (size_t gid = 0; gid < lanes; ++gid) gather_body<float, ZV, 3>(GJ, gid);
+    cpu_build::write_file(dir + "/bp_direct.bin", out);
+  }
   std::printf("H %d w_written %zu w_total %zu blocks %d pieces %d\n", H, w_written, W.size(),
               sched_mode ? grid : npairs * p.C, pieces);
   return 0;

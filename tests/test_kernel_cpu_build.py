@@ -144,11 +144,12 @@ def test_forward_body_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel
 @pytest.mark.parametrize("name,spec,sel,att_mode", [c for c in CASES if c[3] != 2],
                          ids=[c[0] for c in CASES if c[3] != 2])
 def test_balanced_schedule_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel, att_mode, blocks):
-    """The balanced schedule (one-block schedule kernel, blocks walking equal
-    runs of the concatenated plane ranges, the piece reduce) on 1, 5 and 37
-    blocks: pieces of one (angle, tile) pair split across blocks and blocks
-    spanning several pairs.  Every slot the reduce reads is written (the
-    partials start as NaN) and the projections match the oracle."""
+    """The balanced schedule (every block evaluating the schedule and
+    locating its piece, blocks dealt to pieces in the XCD order, the piece
+    reduce) sized for 1, 5 and 37 resident blocks: pairs split into several
+    pieces, grids larger than the resident blocks.  Every slot the reduce
+    reads is written (the partials start as NaN) and the projections match
+    the oracle."""
     spec = dict(spec)
     pa = problem_arrays(spec.pop("n"), spec.pop("n_angles"), seed=5, **spec)
     proj, stats = _run(cpu_build["fwd_cpu"], tmp_path, pa, sel, att_mode=att_mode, sched_blocks=blocks)
@@ -220,6 +221,12 @@ def test_backward_bodies_on_cpu_match_oracle(cpu_build, tmp_path, name, spec, se
           f"reference rel-L2 {e_ref:.1e}, adjoint rel-L2 {e_adj:.1e}")
     assert bool(torch.isfinite(ref_cpu).all()) and bool(torch.isfinite(adj_cpu).all())
     assert e_ref <= 1e-5 and e_adj <= 1e-5
+    if (tmp_path / "bp_direct.bin").exists():
+        # no attenuation, no PSF: the gather of the projection itself
+        direct = torch.from_numpy(np.fromfile(tmp_path / "bp_direct.bin", dtype=np.float32).reshape(shape))
+        e_dir = rel_l2(direct, ref)
+        print(f"{name}: direct reference back projection rel-L2 {e_dir:.1e}")
+        assert bool(torch.isfinite(direct).all()) and e_dir <= 1e-5
 
 
 @pytest.mark.parametrize("blocks", [1, 6, 41])
