@@ -85,13 +85,13 @@ typedef struct spect_geom {
   const float *taps_z;     /* host [n_radii][P][psf_kz] */
   const double *taps_r64;  /* float64 copies (NULL: no float64 PSF) */
   const double *taps_z64;
-  int32_t chunk_planes;    /* planes per work chunk (0: default, 16 planes, and 24 for the
-                              back projection's field kernel in attenuation-field mode;
-                              a positive value applies to every kernel) */
-  int32_t sched_rounds;    /* forward without attenuation or with the attenuation field:
-                              its visited planes are split evenly over sched_rounds x the
-                              blocks the GPU holds at once (0: 1 round); -1: uniform
-                              chunks of chunk_planes planes instead */
+  int32_t chunk_planes;    /* planes per uniform work chunk (0: default, 16 planes) */
+  int32_t sched_rounds;    /* forward and field kernel without attenuation or with the
+                              attenuation field: their visited planes are split evenly over
+                              sched_rounds x the blocks the GPU holds at once; 0: automatic
+                              -- one round when the call's pieces of at most 32 planes fit
+                              the resident blocks, else uniform chunks; -1: always uniform
+                              chunks of chunk_planes planes */
 } spect_geom;
 
 const char *spect_last_error(void);

@@ -23,9 +23,8 @@ struct spect_plan {
   double dx, dy, dz;
   int H;                 // PSF half width used by the kernels (0: none)
   int L, C;              // planes per chunk, chunk count
-  int rounds;            // rounds of the forward's balanced schedule (-1: uniform chunks)
-  int L_field;           // planes per chunk of the back projection's field kernel in
-                         // attenuation-field mode (no state carried across chunks)
+  int rounds;            // rounds of the balanced schedule (0: automatic, -1: uniform chunks)
+  int pieces_max = 0;    // most pieces of kPieceMax planes one angle's tiles need
   int n_radii;
   bool has64;
   int device;
@@ -145,18 +144,30 @@ struct FwdPart {
   int sched_blocks;         // blocks of the scheduled forward (0: not runnable)
 };
 
+// Automatic schedule choice (sched_rounds 0): the balanced schedule when the
+// pieces of an n-angle call fit one round of `blocks` resident blocks even at
+// the longest piece (kPieceMax planes); otherwise -- large problems, where the
+// schedule would run several rounds of kPieceMax-plane pieces with a ragged
+// last round -- uniform chunks, which the hardware dispatcher balances.
+bool sched_auto_ok(const spect_plan *p, int n, int blocks) {
+  if (blocks <= 0) return false;
+  if (p->rounds != 0) return true;
+  return (long long)n * p->pieces_max <= (long long)blocks;
+}
+
 FwdPart fwd_part_layout(const spect_plan *p, int dtype, int n) {
   const size_t rs = dtype == SPECT_F64 ? 8 : 4;
   const Geo<float> g = geo<float>(p);
   const size_t JO = kTJ + 2 * p->H;
   const size_t npairs = (size_t)n * g.ntiles;
   FwdPart l;
-  l.sched_blocks = p->rounds < 0 ? 0
-                   : p->rounds * (dtype == SPECT_F64 ? forward_sched_blocks<double>(geo<double>(p))
-                                                     : forward_sched_blocks<float>(g));
+  const int res = p->rounds < 0 ? 0
+                  : std::max(1, p->rounds) *
+                        (dtype == SPECT_F64 ? forward_sched_blocks<double>(geo<double>(p))
+                                            : forward_sched_blocks<float>(g));
+  l.sched_blocks = sched_auto_ok(p, n, res) ? res : 0;
   const size_t chunk = (size_t)p->C * npairs * JO * p->Nz * rs;
-  const size_t sched =
-      (size_t)sched_grid_blocks(n, g.ntiles, p->P, l.sched_blocks) * JO * p->Nz * rs;
+  const size_t sched = (size_t)sched_grid_blocks(n, g.ntiles, p->P, res) * JO * p->Nz * rs;
   l.sched_off = align256(std::max(chunk, sched));
   l.total = l.sched_off + align256((2 + npairs + 1) * sizeof(int));
   return l;
@@ -303,20 +314,13 @@ int backward_impl(const spect_plan *p, int mode, int out_mode, const R *proj, co
     return SPECT_OK;
   }
   R *W = reinterpret_cast<R *>(ws + l.w_off);
-  // With the attenuation field the field kernel reads per-plane factors and
-  // carries nothing from one chunk to the next, so it runs with its own chunk
-  // length; the suffix-table mode reads rows at the plan's chunk boundaries.
-  Geo<R> gf = g;
-  if (attf) {
-    gf.L = p->L_field;
-    gf.C = (p->P + gf.L - 1) / gf.L;
-  }
-  const int fb = (attf || !mu) && p->rounds >= 0 ? p->rounds * field_sched_blocks<R>(gf) : 0;
-  if (fb > 0) {
+  const int fb = (attf || !mu) && p->rounds >= 0 ? std::max(1, p->rounds) * field_sched_blocks<R>(g)
+                                                  : 0;
+  if (sched_auto_ok(p, n, fb)) {
     // the balanced schedule (as the forward's, for this kernel's blocks)
-    HIP_TRY(launch_bp_field_sched<R>(gf, tables<R>(p), proj, attf, idx, n, W, fb, st));
+    HIP_TRY(launch_bp_field_sched<R>(g, tables<R>(p), proj, attf, idx, n, W, fb, st));
   } else {
-    HIP_TRY(launch_bp_field<R>(gf, tables<R>(p), proj, mu, att, global, attf, idx, n, W, nullptr,
+    HIP_TRY(launch_bp_field<R>(g, tables<R>(p), proj, mu, att, global, attf, idx, n, W, nullptr,
                                nullptr, nullptr, st));
   }
   HIP_TRY(launch_bp_gather<R>(g, tables<R>(p), W, idx, n, mode, out_mode, out, f, nrm, R(delta),
@@ -696,10 +700,7 @@ int spect_plan_create(const spect_geom *g, spect_plan **out) {
   p->H = H;
   p->L = g->chunk_planes > 0 ? g->chunk_planes : 16;
   p->C = (p->P + p->L - 1) / p->L;
-  // Default: 24-plane field chunks (fewer blocks, each keeping its projection
-  // window for more planes); an explicit chunk_planes applies to every kernel.
-  p->L_field = g->chunk_planes > 0 ? g->chunk_planes : 24;
-  p->rounds = g->sched_rounds == 0 ? 1 : g->sched_rounds;
+  p->rounds = g->sched_rounds;
   p->n_radii = g->psf_kr > 0 ? g->n_radii : 1;
   p->has64 = g->angles_deg64 != nullptr && (g->psf_kr == 0 || g->taps_r64 != nullptr);
   (void)hipGetDevice(&p->device);
@@ -753,6 +754,17 @@ int spect_plan_create(const spect_geom *g, spect_plan **out) {
       return cleanup(rc);
     std::vector<int2> r2(rng.size() / 2);
     for (size_t k = 0; k < r2.size(); ++k) r2[k] = make_int2(rng[2 * k], rng[2 * k + 1]);
+    // the most pieces of kPieceMax planes the tiles of one angle split into:
+    // n times this bounds the pieces of any n-angle call at the longest piece
+    const size_t nt = p->A > 0 ? r2.size() / (size_t)p->A : 0;
+    for (int a = 0; a < p->A; ++a) {
+      int pc = 0;
+      for (size_t t = 0; t < nt; ++t) {
+        const int2 r = r2[(size_t)a * nt + t];
+        if (r.y >= r.x) pc += (r.y - r.x + kPieceMax) / kPieceMax;
+      }
+      p->pieces_max = std::max(p->pieces_max, pc);
+    }
     if ((rc = upload(&p->ranges, r2.data(), r2.size()))) return cleanup(rc);
   }
   if ((rc = upload(&p->radius_index, ridx.data(), ridx.size()))) return cleanup(rc);

@@ -57,13 +57,14 @@ class SPECTSystemMatrix(SystemMatrix):
         proj_meta: ``SPECTProjMeta``.
         object_initial_based_on_camera_path: not supported (must be False).
         sensitivity_dtype: dtype of the H^T 1 computation (float64 default).
-        chunk_planes: planes per kernel work unit (0: library default -- 16, and
-            24 for the back projection's field kernel with the attenuation field;
-            a positive value applies to every kernel).
-        sched_rounds: the forward projection without attenuation or with the
-            attenuation field splits its visited planes evenly over this many
-            rounds of the blocks the GPU holds at once (0: library default, 1);
-            -1 uses uniform chunks of ``chunk_planes`` planes instead.
+        chunk_planes: planes per uniform work chunk (0: library default, 16).
+        sched_rounds: the forward projection and the back projection's field
+            kernel, without attenuation or with the attenuation field, split
+            their visited planes evenly over this many rounds of the blocks the
+            GPU holds at once; 0 (default) chooses per call: one round when the
+            call's pieces of at most 32 planes fit the resident blocks, uniform
+            chunks otherwise (large problems); -1 always uses uniform chunks of
+            ``chunk_planes`` planes.
         attenuation_field: keep the attenuation factor of every rotated-frame
             voxel of every angle resident in device memory (built once per
             attenuation map) instead of resampling mu in every projection.
