@@ -1,5 +1,5 @@
 """Batched OSEM over many independent SPECT reconstructions, and the
-autotuner of the kernels' plane-chunk length.
+autotuners of the kernels' plane-chunk length and work schedule.
 
 The reference reconstructs one problem at a time through the Python subset
 loop of ``PreconditionedGradientAscentAlgorithm.__call__``
@@ -135,6 +135,41 @@ def _store_cache() -> None:
         pass
 
 
+def _osem_step_seconds(sm, n: int, n_subsets: int, device, reps: int) -> float:
+    """Mean seconds of one OSEM sub-iteration (subset 0) on unit data, after
+    one untimed call (attenuation data, plan, workspace)."""
+    sm.set_n_subsets(n_subsets)
+    a_s = len(sm.subset_indices_array[0])
+    nz = sm.object_meta.shape[2]
+    f = torch.ones((n, n, nz), device=device)
+    g = torch.ones((a_s, n, nz), device=device)
+    norm = torch.ones_like(f)
+    sm.osem_step(f, g, None, norm, 0, keep_prediction=False)
+    torch.cuda.synchronize(device)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sm.osem_step(f, g, None, norm, 0, keep_prediction=False)
+    torch.cuda.synchronize(device)
+    return (time.perf_counter() - t0) / reps
+
+
+def _tune(key: str, device, candidates, seconds):
+    device = device or pytomography_amd.device
+    cache = _load_cache()
+    ck = f"{torch.cuda.get_device_name(device)}|{key}"
+    if ck in cache:
+        v = cache[ck]
+        return tuple(v) if isinstance(v, list) else int(v)
+    best, best_t = None, float("inf")
+    for cand in candidates:
+        t = seconds(cand, device)
+        if t < best_t:
+            best, best_t = cand, t
+    cache[ck] = list(best) if isinstance(best, tuple) else best
+    _store_cache()
+    return best
+
+
 def autotune_chunk_planes(make_system_matrix, n: int, n_angles: int, n_subsets: int, key: str,
                           device=None, candidates=CHUNK_CANDIDATES, reps: int = 3) -> int:
     """Plane-chunk length (planes per work block) with the fastest OSEM
@@ -142,29 +177,25 @@ def autotune_chunk_planes(make_system_matrix, n: int, n_angles: int, n_subsets: 
 
     ``make_system_matrix(chunk_planes)`` builds a system matrix of the class.
     Results are cached per device name and key."""
-    device = device or pytomography_amd.device
-    cache = _load_cache()
-    name = torch.cuda.get_device_name(device)
-    ck = f"{name}|{key}"
-    if ck in cache:
-        return int(cache[ck])
-    best, best_t = None, float("inf")
-    for cand in candidates:
-        sm = make_system_matrix(cand)
-        sm.set_n_subsets(n_subsets)
-        a_s = len(sm.subset_indices_array[0])
-        f = torch.ones((n, n, n), device=device)
-        g = torch.ones((a_s, n, n), device=device)
-        norm = torch.ones_like(f)
-        sm.osem_step(f, g, None, norm, 0, keep_prediction=False)   # warm-up (att table, plan)
-        torch.cuda.synchronize(device)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            sm.osem_step(f, g, None, norm, 0, keep_prediction=False)
-        torch.cuda.synchronize(device)
-        t = (time.perf_counter() - t0) / reps
-        if t < best_t:
-            best, best_t = cand, t
-    cache[ck] = best
-    _store_cache()
-    return best
+    return _tune(key, device, candidates,
+                 lambda c, dev: _osem_step_seconds(make_system_matrix(c), n, n_subsets, dev, reps))
+
+
+# (chunk_planes, sched_rounds): the library's per-call choice, the balanced
+# schedule forced over 1, 2 or 4 rounds of resident blocks, and uniform chunks
+# of 8, 16 or 32 planes
+SCHEDULE_CANDIDATES = ((0, 0), (0, 1), (0, 2), (0, 4), (8, -1), (16, -1), (32, -1))
+
+
+def autotune_schedule(make_system_matrix, n: int, n_angles: int, n_subsets: int, key: str,
+                      device=None, candidates=SCHEDULE_CANDIDATES, reps: int = 3) -> tuple[int, int]:
+    """Work schedule ``(chunk_planes, sched_rounds)`` of the projector kernels
+    with the fastest OSEM sub-iteration for this problem class, measured on the
+    device: how the (angle, column tile) pairs' visited planes are cut into
+    blocks -- balanced pieces over a number of rounds of resident blocks, or
+    uniform chunks of a plane count.
+
+    ``make_system_matrix(chunk_planes, sched_rounds)`` builds a system matrix
+    of the class.  Results are cached per device name and key."""
+    return _tune("sched|" + key, device, candidates,
+                 lambda c, dev: _osem_step_seconds(make_system_matrix(*c), n, n_subsets, dev, reps))

@@ -25,7 +25,8 @@ def oracle_geometry(pa, dtype=torch.float32, psf_shape="gaussian", psf_dims="2D"
 
 
 def product_system_matrix(pa, device="cuda", chunk_planes=0, attenuation_field=None,
-                          obj2obj=None, proj2proj=(), psf_shape="gaussian", psf_dims="2D"):
+                          obj2obj=None, proj2proj=(), psf_shape="gaussian", psf_dims="2D",
+                          sched_rounds=0):
     """The product system matrix of a problem: obj2obj transforms
     [attenuation?, PSF?] by default, else ``obj2obj`` whose items are "att",
     "psf" or transform objects, in that order."""
@@ -49,7 +50,7 @@ def product_system_matrix(pa, device="cuda", chunk_planes=0, attenuation_field=N
         else:
             tr.append(t)
     return SPECTSystemMatrix(tr, list(proj2proj), om, pm, chunk_planes=chunk_planes,
-                             attenuation_field=attenuation_field)
+                             attenuation_field=attenuation_field, sched_rounds=sched_rounds)
 
 
 def rel_l2(a, b):

@@ -73,6 +73,29 @@ def test_autotuner_returns_candidate(cuda):
     assert best in CHUNK_CANDIDATES
 
 
+def test_schedule_autotuner_returns_candidate(cuda):
+    """The schedule autotuner returns one of its candidates, and every
+    candidate schedule (balanced pieces over 1..4 rounds, uniform chunks of
+    8..32 planes) projects and back-projects as the default one does: the
+    schedules differ only in how the planes' partial sums are grouped."""
+    from pytomography_amd.batch import autotune_schedule, SCHEDULE_CANDIDATES
+    pa = problem_arrays(32, 16, seed=0, att=True, psf=True)
+    make = lambda c, r: product_system_matrix(pa, device=cuda, chunk_planes=c, sched_rounds=r)
+    best = autotune_schedule(make, 32, 16, 4, key="test_32_16_4", device=cuda)
+    assert best in SCHEDULE_CANDIDATES
+    f = torch.from_numpy(pa["f"]).to(cuda)
+    ref = make(0, 0)
+    p_ref = ref.forward(f)
+    b_ref = ref.backward(p_ref)
+    for c, r in SCHEDULE_CANDIDATES:
+        sm = make(c, r)
+        p = sm.forward(f)
+        b = sm.backward(p_ref)
+        print(f"chunk {c} rounds {r}: forward {rel_l2(p, p_ref):.2e} backward {rel_l2(b, b_ref):.2e}")
+        assert rel_l2(p, p_ref) < 1e-6
+        assert rel_l2(b, b_ref) < 1e-6
+
+
 def test_batched_problems_sharing_a_system_matrix(cuda):
     """Several datasets on one scanner geometry: the likelihoods share one
     SPECTSystemMatrix (and its plan and attenuation data); each batched
