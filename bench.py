@@ -13,7 +13,7 @@ kernel (timed with HIP events on its stream) and a CPU baseline (the oracle's
 reference-equivalent torch-CPU path on a bounded sample).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload att_psf|plain|batch|unrolled|aux]
+                    [--workload att_psf|plain|batch|unrolled|aux|schedules]
 """
 from __future__ import annotations
 
@@ -445,12 +445,73 @@ def bench_aux(args, device):
     print(json.dumps(out))
 
 
+def bench_schedules(args, device):
+    """One att+PSF OSEM sub-iteration (subset 0 of 8) per problem size, under
+    the library's per-call schedule choice (sched_rounds 0), the balanced
+    schedule forced to one round (1) and uniform 16-plane chunks (-1); HIP
+    events over ``steps`` back-to-back sub-iterations after ``warmup``."""
+    import pytomography_amd
+    from pytomography_amd import synthetic
+    from pytomography_amd.metadata.SPECT import SPECTObjectMeta, SPECTProjMeta, SPECTPSFMeta
+    from pytomography_amd.transforms.SPECT import SPECTAttenuationTransform, SPECTPSFTransform
+    from pytomography_amd.projectors.SPECT import SPECTSystemMatrix
+
+    pytomography_amd.set_device(device)
+    sizes = [(64, 60), (96, 120), (128, 60), (128, 120), (160, 60), (160, 120), (192, 60), (192, 120)]
+    modes = {"auto": 0, "balanced": 1, "uniform": -1}
+    rows = []
+    for n, na in sizes:
+        mu = torch.from_numpy(synthetic.attenuation_map(n, 0)).to(device)
+        om = SPECTObjectMeta((synthetic.DX,) * 3, (n, n, n))
+        pm = SPECTProjMeta((n, n), (synthetic.DX, synthetic.DX), list(synthetic.angles(na)),
+                           radii=[synthetic.RADIUS] * na)
+        row = {"n": n, "angles": na}
+        for name, rounds in modes.items():
+            print(f"[bench] schedules: {n}^3 x {na}, {name}", file=sys.stderr, flush=True)
+            sm = SPECTSystemMatrix([SPECTAttenuationTransform(mu),
+                                    SPECTPSFTransform(SPECTPSFMeta(synthetic.PSF_PARAMS))], [], om, pm,
+                                   sched_rounds=rounds)
+            sm.set_n_subsets(8)
+            a_s = len(sm.subset_indices_array[0])
+            f = torch.rand((n, n, n), device=device) + 0.5
+            g = torch.rand((a_s, n, n), device=device) + 0.5
+            norm = torch.ones_like(f)
+            for _ in range(max(1, args.warmup)):
+                sm.osem_step(f, g, None, norm, 0, keep_prediction=False)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(args.steps):
+                sm.osem_step(f, g, None, norm, 0, keep_prediction=False)
+            e1.record()
+            torch.cuda.synchronize()
+            row[f"{name}_us"] = 1e3 * e0.elapsed_time(e1) / args.steps
+            del sm, f, g, norm
+        rows.append(row)
+        del mu
+        torch.cuda.empty_cache()
+    best = [min(r["balanced_us"], r["uniform_us"]) for r in rows]
+    out = {
+        "metric": "att+PSF OSEM sub-iteration time per problem size and plane schedule",
+        "value": float(np.mean([r["auto_us"] / b for r, b in zip(rows, best)])),
+        "unit": "mean auto / best-of-two sub-iteration time",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (uniform random object and projections, seeded mu)",
+        "config": {"workload": "one OSEM sub-iteration (subset 0 of 8, attenuation field + PSF) "
+                               "per size: N in {64..192}, A in {60, 120}",
+                   "sizes": rows},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["att_psf", "plain", "batch", "unrolled", "aux"],
+    ap.add_argument("--workload", choices=["att_psf", "plain", "batch", "unrolled", "aux",
+                                           "schedules"],
                     default="att_psf")
     ap.add_argument("--iters", type=int, default=10, help="iterations of the unrolled workload")
     ap.add_argument("--batch", type=int, default=32, help="problems of the batch workload")
@@ -487,6 +548,9 @@ def main():
         return
     if args.workload == "aux":
         bench_aux(args, device)
+        return
+    if args.workload == "schedules":
+        bench_schedules(args, device)
         return
     if args.copy_mib > 0:
         src = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=device).fill_(1.0)
