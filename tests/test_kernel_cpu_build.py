@@ -140,9 +140,20 @@ def test_forward_body_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel
     assert max(errs) <= 1e-5  # the per-projection bar of the GPU parity tests
 
 
-@pytest.mark.parametrize("blocks", [1, 5, 37])
-@pytest.mark.parametrize("name,spec,sel,att_mode", [c for c in CASES if c[3] != 2],
-                         ids=[c[0] for c in CASES if c[3] != 2])
+def _sched_cases(counts):
+    """(case, resident blocks) pairs of the balanced-schedule tests: every
+    block count on the attenuation-field + PSF case, one count per other case
+    (rotating through the counts), so each count runs on several layouts
+    without running every pairing."""
+    sched = [c for c in CASES if c[3] != 2]
+    out = []
+    for k, c in enumerate(sched):
+        for b in (counts if c[0] == "att_field_psf" else [counts[k % len(counts)]]):
+            out.append(pytest.param(*c, b, id=f"{c[0]}-{b}"))
+    return out
+
+
+@pytest.mark.parametrize("name,spec,sel,att_mode,blocks", _sched_cases([1, 5, 37]))
 def test_balanced_schedule_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel, att_mode, blocks):
     """The balanced schedule (every block evaluating the schedule and
     locating its piece, blocks dealt to pieces in the XCD order, the piece
@@ -229,9 +240,7 @@ def test_backward_bodies_on_cpu_match_oracle(cpu_build, tmp_path, name, spec, se
         assert bool(torch.isfinite(direct).all()) and e_dir <= 1e-5
 
 
-@pytest.mark.parametrize("blocks", [1, 6, 41])
-@pytest.mark.parametrize("name,spec,sel,att_mode", [c for c in CASES if c[3] != 2],
-                         ids=[c[0] for c in CASES if c[3] != 2])
+@pytest.mark.parametrize("name,spec,sel,att_mode,blocks", _sched_cases([1, 6, 41]))
 def test_balanced_field_schedule_on_cpu_matches_oracle(cpu_build, tmp_path, name, spec, sel, att_mode,
                                                        blocks):
     """The back-projection field from the balanced schedule (the forward's
