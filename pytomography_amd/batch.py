@@ -170,15 +170,33 @@ def _tune(key: str, device, candidates, seconds):
     return best
 
 
+def _batch_iteration_seconds(sm, n: int, n_angles: int, n_subsets: int, device, reps: int) -> float:
+    """Mean seconds of one OSEM iteration of a one-problem ``BatchedOSEM``
+    (the batched kernels, eager launches) on unit data, after one untimed
+    iteration."""
+    nz = sm.object_meta.shape[2]
+    g = torch.ones((n_angles, n, nz), device=device)
+    batch = BatchedOSEM([PoissonLogLikelihood(sm, g)], n_subsets, use_graph=False)
+    batch.run(1)
+    torch.cuda.synchronize(device)
+    t0 = time.perf_counter()
+    batch.run(reps)
+    torch.cuda.synchronize(device)
+    return (time.perf_counter() - t0) / reps
+
+
 def autotune_chunk_planes(make_system_matrix, n: int, n_angles: int, n_subsets: int, key: str,
                           device=None, candidates=CHUNK_CANDIDATES, reps: int = 3) -> int:
-    """Plane-chunk length (planes per work block) with the fastest OSEM
-    sub-iteration for this problem class, measured on the device.
+    """Plane-chunk length (planes per work block) of the batched kernels
+    (``BatchedOSEM``, which cut every problem into uniform chunks) with the
+    fastest OSEM iteration for this problem class, timed on a one-problem
+    batch on the device.
 
     ``make_system_matrix(chunk_planes)`` builds a system matrix of the class.
     Results are cached per device name and key."""
-    return _tune(key, device, candidates,
-                 lambda c, dev: _osem_step_seconds(make_system_matrix(c), n, n_subsets, dev, reps))
+    return _tune("batch|" + key, device, candidates,
+                 lambda c, dev: _batch_iteration_seconds(make_system_matrix(c), n, n_angles,
+                                                         n_subsets, dev, reps))
 
 
 # (chunk_planes, sched_rounds): the library's per-call choice, the balanced
