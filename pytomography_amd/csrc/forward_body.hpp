@@ -319,43 +319,61 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
       }
     }
   }
-  // taps (and for TRB the contributor lists) of plane i into buffer b; run by tid < tj
+  // bilinear taps of plane i into buffer b; run by tid < tj
   auto setup = [&](int i, int b) {
     R ix, iy;
     rot_coord<R>(cs.x, cs.y, lin_j, ld_uniform(tb.lin + i), g.half, ix, iy);
     tp[b][tid] = taps4<R>(make_src<R>(ix, iy, g.pad), g.N, Nz);
+  };
+  // TRB: the contributor lists of plane i into buffer b, by the whole block
+  // (every thread calls, i block-uniform).  Item q = column * 16 + k tests
+  // object pixel (Xb + k / 4, Yb + k % 4) near the forward sample position of
+  // (i, column); each column's 16 candidates sit in 16 lanes of one wave, and
+  // a ballot packs its contributors in candidate order (X, then Y ascending)
+  // -- at most one entry per candidate, so cnt <= kTrbMax.
+  static_assert(kTrbMax == 16, "one 16-lane group per column");
+  auto trb_lists = [&](int i, int b) {
     if constexpr (TRB) {
-      // object pixels (X, Y) near the forward sample position of (i, j)
-      const int j = j0 + tid;
-      const int Xb = int(floor(iy - R(1.5))), Yb = int(floor(ix - R(1.5)));
-      int cnt = 0;
-      for (int dx_ = 0; dx_ < kTrbSearch; ++dx_) {
-        const int X = Xb + dx_;
-        if ((unsigned)(X - g.pad) >= (unsigned)g.N) continue;
-        for (int dy_ = 0; dy_ < kTrbSearch; ++dy_) {
-          const int Y = Yb + dy_;
-          if ((unsigned)(Y - g.pad) >= (unsigned)g.N) continue;
-          R jc, ic;
-          rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, jc, ic);
-          const R fjc = floor(jc), fic = floor(ic);
-          const int c0 = int(fjc), r0 = int(fic);
-          const R tx = jc - fjc, ty = ic - fic;
-          R wr, wc;
-          if (r0 == i) wr = R(1) - ty;
-          else if (r0 + 1 == i) wr = ty;
-          else continue;
-          if (c0 == j) wc = R(1) - tx;
-          else if (c0 + 1 == j) wc = tx;
-          else continue;
-          // cnt < kTrbMax: at most one entry per visited (X, Y)
-          trb_off[b][tid][cnt] = (uint32_t)((X - g.pad) * g.N + (Y - g.pad)) * (uint32_t)Nz;
-          trb_w[b][tid][cnt] = wr * wc;
-          ++cnt;
+      const int nitems = tj * kTrbMax;
+      const R lin_i = ld_uniform(tb.lin + i);
+      const unsigned lane = tid & 63u;
+      for (int q0 = 0; q0 < nitems; q0 += blockDim.x) {
+        const int q = q0 + tid;
+        const int col = q >> 4, k = q & 15;
+        bool use = false;
+        uint32_t off = 0;
+        R w = R(0);
+        if (q < nitems) {
+          R ix, iy;
+          rot_coord<R>(cs.x, cs.y, tb.lin[j0 + col], lin_i, g.half, ix, iy);
+          const int X = int(floor(iy - R(1.5))) + (k >> 2), Y = int(floor(ix - R(1.5))) + (k & 3);
+          if ((unsigned)(X - g.pad) < (unsigned)g.N && (unsigned)(Y - g.pad) < (unsigned)g.N) {
+            R jc, ic;
+            rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, jc, ic);
+            const R fjc = floor(jc), fic = floor(ic);
+            const int c0 = int(fjc), r0 = int(fic);
+            const R tx = jc - fjc, ty = ic - fic;
+            const int j = j0 + col;
+            const bool rok = r0 == i || r0 + 1 == i, cok = c0 == j || c0 + 1 == j;
+            if (rok && cok) {
+              const R wr = r0 == i ? R(1) - ty : ty;
+              const R wc = c0 == j ? R(1) - tx : tx;
+              use = true;
+              off = (uint32_t)((X - g.pad) * g.N + (Y - g.pad)) * (uint32_t)Nz;
+              w = wr * wc;
+            }
+          }
         }
+        const unsigned grp = (unsigned)(__ballot(use) >> (lane & 48u)) & 0xFFFFu;
+        if (use) {
+          const int slot = __popc(grp & ((1u << (lane & 15u)) - 1u));
+          trb_off[b][col][slot] = off;
+          trb_w[b][col][slot] = w;
+        }
+        // a contributor's forward sample lies within sqrt(2) px of the object,
+        // inside the footprint square, so the plane is in act
+        if (q < nitems && k == 0) trb_n[b][col] = __popc(grp);
       }
-      // a contributor's forward sample lies within sqrt(2) px of the object,
-      // inside the footprint square, so the plane is in act
-      trb_n[b][tid] = cnt;
     }
   };
   __syncthreads();
@@ -364,6 +382,7 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
   if (i >= 0) {
     rem &= ~(1ull << (i - i_lo));
     if (tid < tj) setup(i, 0);
+    trb_lists(i, 0);
   }
   __syncthreads();
   // PF: the attenuation factors of the next plane (a coalesced stream from
@@ -533,7 +552,10 @@ __device__ __forceinline__ void fwd_body(const FwdJob<R> &J, int al, int jt, con
       if (zvec) phase_a(std::true_type{});
       else phase_a(std::false_type{});
     }
-    if (inext >= 0 && tid < tj) setup(inext, b ^ 1);
+    if (inext >= 0) {
+      if (tid < tj) setup(inext, b ^ 1);
+      trb_lists(inext, b ^ 1);
+    }
     if constexpr (RAY) {
       // the next plane's taps are complete; this plane's may be rewritten
       __syncthreads();

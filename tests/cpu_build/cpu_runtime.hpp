@@ -80,6 +80,17 @@ int __builtin_amdgcn_readlane(int v, int lane) {
   return r;
 }
 
+unsigned long long __ballot(int pred) {
+  const unsigned t = threadIdx.x, w0 = t & ~63u;
+  g_shfl_slot[t] = pred ? 1 : 0;
+  cpu_block_barrier->arrive_and_wait();
+  unsigned long long m = 0;
+  for (unsigned l = 0; l < 64 && w0 + l < blockDim.x; ++l)
+    if (g_shfl_slot[w0 + l]) m |= 1ull << l;
+  cpu_block_barrier->arrive_and_wait();
+  return m;
+}
+
 namespace cpu_build {
 
 template <typename T> std::vector<T> read_file(const std::string &path, size_t count) {

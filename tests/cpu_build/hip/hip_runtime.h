@@ -47,6 +47,10 @@ int __builtin_amdgcn_update_dpp(int old, int src, int dpp_ctrl, int row_mask, in
                                 bool bound_ctrl);
 int __builtin_amdgcn_readlane(int v, int lane);
 inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
+// Bit l of the result: pred of lane l of the caller's wave (lanes past
+// blockDim.x read as 0).
+unsigned long long __ballot(int pred);
+inline int __popc(unsigned v) { return __builtin_popcount(v); }
 
 struct float2 {
   float x, y;
