@@ -76,6 +76,14 @@ __global__ __launch_bounds__(256) void bp_gather_kernel(GatherJob<R> J) {
   gather_body<R, ZV, MODE>(J, (size_t)blockIdx.x * 256 + threadIdx.x);
 }
 
+// Exact-adjoint modes with the block's contributor lists in LDS
+// (gather_adj_block; dynamic LDS adj_block_smem).
+template <typename R, int ZV, int MODE>
+__global__ __launch_bounds__(256) void bp_gather_adj_kernel(GatherJob<R> J) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  gather_adj_block<R, ZV, MODE>(J, blockIdx.x, smem);
+}
+
 template <typename R, int ZV, int MODE>
 __global__ __launch_bounds__(256) void gather_batch_kernel(const GatherJob<R> *__restrict__ jobs,
                                                            const int *__restrict__ starts,
@@ -242,10 +250,13 @@ hipError_t launch_bp_gather(const Geo<R> &g, const Tables<R> &tb, const R *W,
     bp_gather_kernel<R, ZV, 3><<<blocks, 256, 0, st>>>(J);  // rows alias one projection row
   else if (mode == SPECT_BP_REFERENCE)
     bp_gather_kernel<R, ZV, 0><<<blocks, 256, 0, st>>>(J);
-  else if (mode == SPECT_BP_ADJOINT)
-    bp_gather_kernel<R, ZV, 1><<<blocks, 256, 0, st>>>(J);
-  else
-    bp_gather_kernel<R, ZV, 2><<<blocks, 256, 0, st>>>(J);
+  else {
+    const size_t smem = adj_block_smem<R>((g.Nz + ZV - 1) / ZV);  // <= 50 KB (Nz <= ZV)
+    if (mode == SPECT_BP_ADJOINT)
+      bp_gather_adj_kernel<R, ZV, 1><<<blocks, 256, smem, st>>>(J);
+    else
+      bp_gather_adj_kernel<R, ZV, 2><<<blocks, 256, smem, st>>>(J);
+  }
   return hipGetLastError();
 }
 

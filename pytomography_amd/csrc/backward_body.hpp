@@ -540,4 +540,154 @@ __device__ __forceinline__ void gather_body(const GatherJob<R> &J, size_t gid) {
   }
 }
 
+// Dynamic LDS of gather_adj_block: per column of the block a contributor
+// count and 16 (row << 16 | column, weight) entries.
+template <typename R> __host__ __device__ inline int adj_block_cols(int zq) { return 256 / zq + 2; }
+template <typename R> __host__ __device__ inline size_t adj_block_smem(int zq) {
+  return (size_t)adj_block_cols<R>(zq) * (sizeof(int) + 16 * (sizeof(uint32_t) + sizeof(R)));
+}
+
+// Exact-adjoint gather (MODE 1; MODE 2 adds the mu-gradient sums) of block b
+// of a 256-lane launch over the lanes of gather_body.  Per angle, the block
+// first builds the contributor list of each of its object columns once -- the
+// rotated-frame pixels (i, j) whose forward bilinear sample holds the
+// column's pixel, tested over the 4 x 4 candidates of gather_body's search by
+// one 16-lane group per column and packed by a wave ballot in candidate order
+// -- and every z lane of the column then sums its list in that order: the
+// same sums as gather_body, without each of a column's z lanes repeating the
+// search.  Every thread of the block calls (barriers).
+template <typename R, int ZV, int MODE>
+__device__ __forceinline__ void gather_adj_block(const GatherJob<R> &J, int b, unsigned char *smem) {
+  static_assert(MODE == 1 || MODE == 2, "exact-adjoint modes");
+  const Geo<R> &g = J.g;
+  const Tables<R> &tb = J.tb;
+  const R *__restrict__ W = J.W;
+  const int Nz = g.Nz, N = g.N;
+  const int zq = (Nz + ZV - 1) / ZV;
+  const size_t total = (size_t)N * N * zq;
+  const size_t gid0 = (size_t)b * 256;
+  const int tid = threadIdx.x;
+  const size_t gid = gid0 + tid;
+  const bool lane_ok = gid < total;
+  const int col_lo = (int)(gid0 / zq);
+  const size_t last = gid0 + 255 < total - 1 ? gid0 + 255 : total - 1;
+  const int ncols = (int)(last / zq) - col_lo + 1;
+  const int cols_max = adj_block_cols<R>(zq);
+  int *cnt = reinterpret_cast<int *>(smem);                    // [cols_max]
+  uint32_t *ent_ij = reinterpret_cast<uint32_t *>(cnt + cols_max);  // [cols_max][16]
+  R *ent_w = reinterpret_cast<R *>(ent_ij + (size_t)cols_max * 16);  // [cols_max][16]
+  const int col = lane_ok ? (int)(gid / zq) : col_lo;
+  const int lc = col - col_lo;
+  const int z0 = lane_ok ? (int)(gid - (size_t)col * zq) * ZV : 0;
+  const int nvalid = min(ZV, Nz - z0);
+  const bool vec = (nvalid == ZV) && (Nz % 4 == 0);
+  const int x = col / N, y = col - x * N;
+  const int n_jext = g.je1 - g.je0;
+  const size_t plane = (size_t)n_jext * Nz;
+  const size_t astride = J.w_astride ? J.w_astride : (size_t)g.P * plane;
+  const size_t rstride = J.w_astride ? J.w_rstride : plane;
+  R acc[ZV];
+  double accm[MODE == 2 ? ZV : 1];
+#pragma unroll
+  for (int m = 0; m < ZV; ++m) acc[m] = R(0);
+#pragma unroll
+  for (int m = 0; m < (MODE == 2 ? ZV : 1); ++m) accm[m] = 0.0;
+  const unsigned lane = tid & 63u;
+  for (int al = 0; al < J.n; ++al) {
+    const int ang = ld_uniform(J.angle_idx + al);
+    if ((unsigned)ang >= (unsigned)g.A) continue;  // block-uniform
+    const auto cs = ld_uniform(tb.trig + ang);
+    // ---- the block's columns' lists (item q: column q / 16, candidate q % 16)
+    for (int q0 = 0; q0 < ncols * 16; q0 += 256) {
+      const int q = q0 + tid;
+      const int qc = q >> 4, k = q & 15;
+      bool use = false;
+      uint32_t pij = 0;
+      R w = R(0);
+      if (q < ncols * 16) {
+        const int c = col_lo + qc;
+        const int cx = c / N, cy = c - cx * N;
+        const int X = cx + g.pad, Y = cy + g.pad;
+        // rotated-frame position whose forward sample lands on (row X, col Y)
+        R jc, ic;
+        rot_coord<R>(cs.x, -cs.y, tb.lin[Y], tb.lin[X], g.half, jc, ic);
+        const int i = int(floor(ic - R(1.5))) + (k >> 2), j = int(floor(jc - R(1.5))) + (k & 3);
+        if ((unsigned)i < (unsigned)g.P && j >= g.je0 && j < g.je1) {
+          R ix, iy;
+          rot_coord<R>(cs.x, cs.y, tb.lin[j], tb.lin[i], g.half, ix, iy);
+          const R fx = floor(ix), fy = floor(iy);
+          const int c0 = int(fx), r0 = int(fy);
+          const R tx = ix - fx, ty = iy - fy;
+          if ((r0 == X || r0 + 1 == X) && (c0 == Y || c0 + 1 == Y)) {
+            const R wr = r0 == X ? R(1) - ty : ty;
+            const R wc = c0 == Y ? R(1) - tx : tx;
+            use = true;
+            pij = ((uint32_t)i << 16) | (uint32_t)(j - g.je0);
+            w = wr * wc;
+          }
+        }
+      }
+      const unsigned grp = (unsigned)(__ballot(use) >> (lane & 48u)) & 0xFFFFu;
+      if (use) {
+        const int slot = __popc(grp & ((1u << (lane & 15u)) - 1u));
+        ent_ij[(size_t)qc * 16 + slot] = pij;
+        ent_w[(size_t)qc * 16 + slot] = w;
+      }
+      if (q < ncols * 16 && k == 0) cnt[qc] = __popc(grp);
+    }
+    __syncthreads();
+    // ---- the z lanes sum their column's list
+    if (lane_ok) {
+      const R *Wa = W + (size_t)al * astride;
+      R v[ZV];
+#pragma unroll
+      for (int m = 0; m < ZV; ++m) v[m] = R(0);
+      const int nc = cnt[lc];
+      auto sum = [&](auto vec_c) {
+        constexpr bool VEC = decltype(vec_c)::value;
+        for (int e = 0; e < nc; ++e) {
+          const uint32_t pij = ent_ij[(size_t)lc * 16 + e];
+          const R w = ent_w[(size_t)lc * 16 + e];
+          const int i = (int)(pij >> 16), jr = (int)(pij & 0xFFFFu);
+          const R *p = Wa + (size_t)i * rstride + (size_t)jr * Nz + z0;
+          R t[ZV];
+          if constexpr (VEC) {
+            const ZVec<R, ZV> tv = load_zvec<R, ZV>(p);
+#pragma unroll
+            for (int m = 0; m < ZV; ++m) t[m] = tv.v[m];
+          } else {
+#pragma unroll
+            for (int m = 0; m < ZV; ++m) t[m] = m < nvalid ? p[m] : R(0);
+          }
+#pragma unroll
+          for (int m = 0; m < ZV; ++m) v[m] = v[m] + t[m] * w;
+          if constexpr (MODE == 2) {
+            const size_t o = (size_t)i * plane + (size_t)jr * Nz + z0;
+            const double *wm = J.Wm + (size_t)al * g.P * plane + o;
+            const double *pr = J.Pref + ((size_t)al * g.C + i / g.L) * plane + (size_t)jr * Nz + z0;
+#pragma unroll
+            for (int m = 0; m < ZV; ++m)
+              if (m < nvalid) accm[m] += (double)w * (wm[m] - pr[m]);
+          }
+        }
+      };
+      if (vec) sum(std::true_type{});
+      else sum(std::false_type{});
+#pragma unroll
+      for (int m = 0; m < ZV; ++m) acc[m] = acc[m] + v[m];
+    }
+    __syncthreads();  // the lists are rebuilt for the next angle
+  }
+  if (!lane_ok) return;
+  const size_t o = ((size_t)x * N + y) * Nz + z0;
+  if (vec) bp_store_vec<R, ZV, true>(J.out_mode, J.out, o, acc, nvalid, J.f, J.nrm, J.delta, J.floor_value);
+  else bp_store_vec<R, ZV, false>(J.out_mode, J.out, o, acc, nvalid, J.f, J.nrm, J.delta, J.floor_value);
+  if constexpr (MODE == 2) {
+    R gm[ZV];
+#pragma unroll
+    for (int m = 0; m < ZV; ++m) gm[m] = R((double)g.dx * accm[m]);
+    bp_store_vec<R, ZV>(J.out_mode, J.out_mu, o, gm, nvalid, J.f, J.nrm, J.delta, J.floor_value);
+  }
+}
+
 }  // namespace spect

@@ -88,8 +88,12 @@ int run_grad(const std::string &dir, const cpu_build::Problem &p, const std::vec
   std::vector<float> grad_mu(grad_f.size(), std::numeric_limits<float>::quiet_NaN());
   const GatherJob<float> GJ{g, tb, W.data(), p.sel.data(), n, SPECT_OUT_STORE, grad_f.data(), nullptr,
                             nullptr, 0.0f, 0.0f, Wm.data(), Dsum.data(), grad_mu.data()};
-  const size_t lanes = (size_t)N * N * ((Nz + ZV - 1) / ZV);
-  for (size_t gid = 0; gid < lanes; ++gid) gather_body<float, ZV, 2>(GJ, gid);
+  // as bp_gather_adj_kernel: 256-lane blocks sharing their columns' lists
+  const int zq = (Nz + ZV - 1) / ZV;
+  const size_t lanes = (size_t)N * N * zq;
+  std::vector<unsigned char> asmem(adj_block_smem<float>(zq));
+  cpu_build::run_grid((int)((lanes + 255) / 256), 1, 1, 256, asmem,
+                      [&](int b, int, int, unsigned char *s) { gather_adj_block<float, ZV, 2>(GJ, b, s); });
   cpu_build::write_file(dir + "/grad_f.bin", grad_f);
   cpu_build::write_file(dir + "/grad_mu.bin", grad_mu);
   std::printf("H %d grad 1\n", H);
@@ -172,16 +176,24 @@ int main(int argc, char **argv) {
   }
   const size_t w_written = W.size() - cpu_build::count_nan(W);
 
-  // launch_bp_gather: one lane per (x, y, ZV z); no barriers
+  // launch_bp_gather: one lane per (x, y, ZV z); the reference mode has no
+  // barriers, the exact adjoint runs as 256-lane blocks sharing their
+  // columns' contributor lists (bp_gather_adj_kernel)
   constexpr int ZV = gather_zv<float>();  // as launch_bp_gather
-  const size_t lanes = (size_t)N * N * ((Nz + ZV - 1) / ZV);
+  const int zq = (Nz + ZV - 1) / ZV;
+  const size_t lanes = (size_t)N * N * zq;
   for (int mode = 0; mode < 2; ++mode) {
     std::vector<float> out((size_t)N * N * Nz, std::numeric_limits<float>::quiet_NaN());
     const GatherJob<float> GJ{g, p.tables(), W.data(), p.sel.data(), n, SPECT_OUT_STORE, out.data(),
                               nullptr, nullptr, 0.0f, 0.0f, nullptr, nullptr, nullptr};
-    for (size_t gid = 0; gid < lanes; ++gid) {
-      if (mode == 0) gather_body<float, ZV, 0>(GJ, gid);
-      else gather_body<float, ZV, 1>(GJ, gid);
+    if (mode == 0) {
+      for (size_t gid = 0; gid < lanes; ++gid) gather_body<float, ZV, 0>(GJ, gid);
+    } else {
+      std::vector<unsigned char> asmem(adj_block_smem<float>(zq));
+      const int gblocks = (int)((lanes + 255) / 256);
+      cpu_build::run_grid(gblocks, 1, 1, 256, asmem, [&](int b, int, int, unsigned char *s) {
+        gather_adj_block<float, ZV, 1>(GJ, b, s);
+      });
     }
     cpu_build::write_file(dir + (mode == 0 ? "/bp_ref.bin" : "/bp_adj.bin"), out);
   }
