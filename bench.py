@@ -686,9 +686,12 @@ def measure_osem(n: int, na: int, ns: int, att: bool, psf: bool, device, steps: 
     del f_tmp, ratios
     a_s = na // ns
     fwd_b, bwd_b, upd_b, ratio_b = algorithmic_bytes(n, a_s, att)
-    # Dominant kernel: the forward projection (fwd_kernel + its chunk reduce),
-    # timed above with HIP events on the stream it is launched on.
-    kern = f"fwd_sched_kernel + reduce_sched_kernel (spect_forward, {a_s} angles)"
+    # Dominant kernel: the forward projection (its projector kernel + the
+    # partials' reduce), timed above with HIP events on the stream it is
+    # launched on; which pair runs follows the plane schedule of the call
+    kern = (f"fwd_sched_kernel + reduce_sched_kernel (spect_forward, {a_s} angles)"
+            if sm.schedule(0, torch.float32)["forward"] == "balanced"
+            else f"fwd_kernel + fwd_reduce_kernel (spect_forward, {a_s} angles)")
     achieved = fwd_b / (fwd_ms * 1e-3) / 1e9
     iter_bytes = ns * (fwd_b + bwd_b + upd_b + ratio_b)
 
@@ -737,6 +740,8 @@ def measure_osem(n: int, na: int, ns: int, att: bool, psf: bool, device, steps: 
                     "note": ("read as a coalesced stream instead of resampling mu per call; the "
                              "forward's measured HBM bytes include it (traffic vs algorithmic bytes "
                              "in the roofline entry)")}
+    # the plane schedule the timed kernels ran (spect_schedule_info)
+    schedule = sm.schedule(0, torch.float32)
     del alg, lk, prob, sm
     torch.cuda.empty_cache()
     return dict(
@@ -751,6 +756,7 @@ def measure_osem(n: int, na: int, ns: int, att: bool, psf: bool, device, steps: 
             "iterations_compared": n_done,
             "forward_ms_per_subset": fwd_ms,
             "backward_update_ms_per_subset": bwd_ms,
+            "schedule": schedule,
             **({"attenuation_data": att_data} if att_data else {}),
         })
 

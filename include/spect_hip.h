@@ -107,6 +107,14 @@ int spect_plan_dims(const spect_plan *p, int32_t *padded, int32_t *pad, int32_t 
 /* Bytes of workspace needed by forward / backward / osem_subiteration over n angles. */
 size_t spect_workspace_bytes(const spect_plan *p, int dtype, int n);
 
+/* The plane schedule an n-angle call runs with this plan, dtype and the
+ * device current now (sched_rounds of spect_geom): *forward = 1 when the
+ * forward projection (no attenuation, or the attenuation field) runs the
+ * balanced schedule, 0 for uniform chunks; *field likewise for the back
+ * projection's field kernel.  No counterpart in the reference: a diagnostic of
+ * this library's launch choice, for callers that tune or report it. */
+int spect_schedule_info(const spect_plan *p, int dtype, int n, int32_t *forward, int32_t *field);
+
 /* Attenuation table: FP64 suffix sums of mu_rot*dx at every chunk boundary
  * for every angle -- the reverse cumsum of utils/misc.py:18 evaluated once
  * per attenuation map (attenuation.py:87-91 recomputes it on every call).

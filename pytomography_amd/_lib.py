@@ -24,7 +24,8 @@ NOALLOC_SCRATCH_BYTES = 128 << 20  # SPECT_NOALLOC_SCRATCH_BYTES
 
 EXPORTED = (
     "spect_last_error", "spect_version", "spect_plan_create", "spect_plan_destroy",
-    "spect_plan_dims", "spect_workspace_bytes", "spect_att_table_bytes", "spect_att_table_build",
+    "spect_plan_dims", "spect_workspace_bytes", "spect_schedule_info",
+    "spect_att_table_bytes", "spect_att_table_build",
     "spect_att_field_bytes", "spect_att_field_build",
     "spect_forward", "spect_forward_noalloc", "spect_backward", "spect_backward_transpose",
     "spect_osem_subiteration",
@@ -89,6 +90,8 @@ def _declare(lib):
     lib.spect_plan_dims.argtypes = [vp] + [ctypes.POINTER(i32)] * 4
     lib.spect_workspace_bytes.restype = sz
     lib.spect_workspace_bytes.argtypes = [vp, ci, ci]
+    lib.spect_schedule_info.restype = ci
+    lib.spect_schedule_info.argtypes = [vp, ci, ci, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.spect_att_table_bytes.restype = sz
     lib.spect_att_table_bytes.argtypes = [vp]
     lib.spect_att_table_build.restype = ci

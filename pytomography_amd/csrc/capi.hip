@@ -826,6 +826,20 @@ size_t spect_workspace_bytes(const spect_plan *p, int dtype, int n) {
   return ws_layout(p, dtype, n).total;
 }
 
+int spect_schedule_info(const spect_plan *p, int dtype, int n, int32_t *forward, int32_t *field) {
+  if (!p) return fail(SPECT_E_ARG, "null plan");
+  if (!forward || !field) return fail(SPECT_E_ARG, "null output argument");
+  if (dtype != SPECT_F32 && dtype != SPECT_F64) return fail(SPECT_E_ARG, "dtype must be F32 or F64");
+  if (n <= 0 || n > p->A) return fail(SPECT_E_ARG, "angle count out of range");
+  // the same choices as forward_core and backward_impl
+  *forward = fwd_part_layout(p, dtype, n).sched_blocks > 0 ? 1 : 0;
+  const int per = dtype == SPECT_F64 ? field_sched_blocks<double>(geo<double>(p))
+                                     : field_sched_blocks<float>(geo<float>(p));
+  const int fb = p->rounds >= 0 ? std::max(1, p->rounds) * per : 0;
+  *field = sched_auto_ok(p, n, fb) ? 1 : 0;
+  return SPECT_OK;
+}
+
 size_t spect_att_table_bytes(const spect_plan *p) {
   if (!p) return 0;
   const Geo<float> g = geo<float>(p);

@@ -21,6 +21,7 @@ Extras beyond the reference:
 """
 from __future__ import annotations
 
+import ctypes
 import inspect
 
 import torch
@@ -143,6 +144,27 @@ class SPECTSystemMatrix(SystemMatrix):
         if ws is None or ws.numel() < nbytes:
             self._ws[key] = ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
         return ws, ws.numel()
+
+    def schedule(self, subset_idx: int | None = None, dtype: torch.dtype | None = None) -> dict:
+        """The plane schedule the fused kernels run for this subset's angles
+        (``spect_schedule_info``): ``{"forward": ..., "field": ...}`` with
+        "balanced" (pieces of the visited planes over one round of resident
+        blocks) or "uniform" (``chunk_planes``-plane chunks); "forward" is the
+        forward projection without attenuation or with the attenuation field,
+        "field" the back projection's field kernel ("none" without attenuation
+        and PSF, where the back projection gathers the projection directly)."""
+        self._require_fused("schedule")
+        device = pytomography_amd.device
+        dt = dtype or (pytomography_amd.dtype if pytomography_amd.dtype == torch.float64
+                       else torch.float32)
+        plan = self._plan(device)
+        n = int(self._angle_index(subset_idx, device).numel())
+        fwd, fld = ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(plan.lib.spect_schedule_info(plan.handle, _lib.dtype_code(dt), n,
+                                                ctypes.byref(fwd), ctypes.byref(fld)))
+        name = lambda v: "balanced" if v.value else "uniform"
+        direct = self._att is None and self._psf is None
+        return {"forward": name(fwd), "field": "none" if direct else name(fld)}
 
     def _angle_index(self, subset_idx, device):
         key = (subset_idx, device.type, device.index)

@@ -28,6 +28,9 @@ def test_automatic_schedule_matches_the_one_it_picks(n, n_angles, picked):
     for rounds in (0, 1, -1):
         sm = product_system_matrix(pa, sched_rounds=rounds)
         sm.set_n_subsets(8)
+        # spect_schedule_info reports the choice the launches make
+        want = "balanced" if (picked if rounds == 0 else rounds) == 1 else "uniform"
+        assert sm.schedule(2) == {"forward": want, "field": want}
         p = sm.forward(f, subset_idx=2)
         b = sm.backward(p, subset_idx=2)
         out[rounds] = (p, b)

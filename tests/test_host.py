@@ -47,6 +47,8 @@ def test_c_abi_rejects_bad_arguments_without_touching_the_gpu():
     assert b"odd" in lib.spect_last_error()
     assert lib.spect_forward(None, 0, None, None, None, None, 1, None, None, 0, None) == -1
     assert lib.spect_workspace_bytes(None, 0, 4) == 0
+    fwd, fld = ctypes.c_int32(), ctypes.c_int32()
+    assert lib.spect_schedule_info(None, 0, 4, ctypes.byref(fwd), ctypes.byref(fld)) == -1
     assert lib.spect_poisson_ratio(0, None, None, None, None, None, 10, 1e-11, None) == -1
     # argument checks of the auxiliary entry points run before any launch
     fake = ctypes.c_void_p(0x1000)
