@@ -90,7 +90,8 @@ typedef struct spect_geom {
                               attenuation field: their visited planes are split evenly over
                               sched_rounds x the blocks the GPU holds at once; 0: automatic
                               -- one round when the call's pieces of at most 32 planes fit
-                              the resident blocks, else uniform chunks; -1: always uniform
+                              the resident blocks (the forward: also at most 19 planes per
+                              block on average), else uniform chunks; -1: always uniform
                               chunks of chunk_planes planes */
 } spect_geom;
 

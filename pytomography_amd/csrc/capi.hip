@@ -25,6 +25,7 @@ struct spect_plan {
   int L, C;              // planes per chunk, chunk count
   int rounds;            // rounds of the balanced schedule (0: automatic, -1: uniform chunks)
   int pieces_max = 0;    // most pieces of kPieceMax planes one angle's tiles need
+  int planes_max = 0;    // most visited planes one angle's tiles hold
   int n_radii;
   bool has64;
   int device;
@@ -148,12 +149,20 @@ struct FwdPart {
 // pieces of an n-angle call fit one round of `blocks` resident blocks even at
 // the longest piece (kPieceMax planes); otherwise -- large problems, where the
 // schedule would run several rounds of kPieceMax-plane pieces with a ragged
-// last round -- uniform chunks, which the hardware dispatcher balances.
-bool sched_auto_ok(const spect_plan *p, int n, int blocks) {
+// last round -- uniform chunks, which the hardware dispatcher balances.  With
+// max_q > 0 the call's planes must also average at most max_q per block: one
+// round of long pieces cannot even out their different tap tiers.
+bool sched_auto_ok(const spect_plan *p, int n, int blocks, int max_q = 0) {
   if (blocks <= 0) return false;
   if (p->rounds != 0) return true;
-  return (long long)n * p->pieces_max <= (long long)blocks;
+  if ((long long)n * p->pieces_max > (long long)blocks) return false;
+  return max_q <= 0 || (long long)n * p->planes_max <= (long long)max_q * blocks;
 }
+
+// The forward's bound on planes per block for the automatic choice: measured
+// with the balanced schedule ahead at up to 15.9 planes per block (160^3, 8
+// angles) and behind at 22.2 (192^3, 8 angles).
+constexpr int kFwdAutoPlanesPerBlock = 19;
 
 FwdPart fwd_part_layout(const spect_plan *p, int dtype, int n) {
   const size_t rs = dtype == SPECT_F64 ? 8 : 4;
@@ -165,7 +174,7 @@ FwdPart fwd_part_layout(const spect_plan *p, int dtype, int n) {
                   : std::max(1, p->rounds) *
                         (dtype == SPECT_F64 ? forward_sched_blocks<double>(geo<double>(p))
                                             : forward_sched_blocks<float>(g));
-  l.sched_blocks = sched_auto_ok(p, n, res) ? res : 0;
+  l.sched_blocks = sched_auto_ok(p, n, res, kFwdAutoPlanesPerBlock) ? res : 0;
   const size_t chunk = (size_t)p->C * npairs * JO * p->Nz * rs;
   const size_t sched = (size_t)sched_grid_blocks(n, g.ntiles, p->P, res) * JO * p->Nz * rs;
   l.sched_off = align256(std::max(chunk, sched));
@@ -758,12 +767,16 @@ int spect_plan_create(const spect_geom *g, spect_plan **out) {
     // n times this bounds the pieces of any n-angle call at the longest piece
     const size_t nt = p->A > 0 ? r2.size() / (size_t)p->A : 0;
     for (int a = 0; a < p->A; ++a) {
-      int pc = 0;
+      int pc = 0, pl = 0;
       for (size_t t = 0; t < nt; ++t) {
         const int2 r = r2[(size_t)a * nt + t];
-        if (r.y >= r.x) pc += (r.y - r.x + kPieceMax) / kPieceMax;
+        if (r.y >= r.x) {
+          pc += (r.y - r.x + kPieceMax) / kPieceMax;
+          pl += r.y - r.x + 1;
+        }
       }
       p->pieces_max = std::max(p->pieces_max, pc);
+      p->planes_max = std::max(p->planes_max, pl);
     }
     if ((rc = upload(&p->ranges, r2.data(), r2.size()))) return cleanup(rc);
   }

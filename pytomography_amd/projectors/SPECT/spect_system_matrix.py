@@ -63,8 +63,9 @@ class SPECTSystemMatrix(SystemMatrix):
             kernel, without attenuation or with the attenuation field, split
             their visited planes evenly over this many rounds of the blocks the
             GPU holds at once; 0 (default) chooses per call: one round when the
-            call's pieces of at most 32 planes fit the resident blocks, uniform
-            chunks otherwise (large problems); -1 always uses uniform chunks of
+            call's pieces of at most 32 planes fit the resident blocks (for the
+            forward also at most 19 planes per block on average), uniform chunks
+            otherwise (large problems); -1 always uses uniform chunks of
             ``chunk_planes`` planes.
         attenuation_field: keep the attenuation factor of every rotated-frame
             voxel of every angle resident in device memory (built once per
