@@ -112,7 +112,7 @@ class BatchedOSEM:
 # ---------------------------------------------------------------------------
 _CACHE_PATH = os.path.join(os.path.expanduser("~"), ".cache", "pytomography_amd", "autotune.json")
 _cache: dict | None = None
-CHUNK_CANDIDATES = (8, 12, 16, 24, 32)
+CHUNK_CANDIDATES = (4, 8, 12, 16, 24, 32)
 
 
 def _load_cache() -> dict:
