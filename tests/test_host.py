@@ -220,3 +220,30 @@ def test_cutoff_transform_pads_and_follows_subset_angles():
     assert torch.equal(t.forward(proj), t.mask)
     sub = torch.tensor([1, 4])
     assert torch.equal(t.backward(proj[:2], sub), t.mask[sub])
+
+
+def test_tuner_picks_the_fastest_candidate_and_caches_it(tmp_path, monkeypatch):
+    """batch._tune (behind autotune_chunk_planes and autotune_schedule): the
+    candidate with the smallest measured time wins, tuple candidates survive
+    the JSON cache, and a cached key is answered without timing again."""
+    from pytomography_amd import batch
+    monkeypatch.setattr(batch, "_CACHE_PATH", str(tmp_path / "autotune.json"))
+    monkeypatch.setattr(batch, "_cache", None)
+    monkeypatch.setattr(batch.torch.cuda, "get_device_name", lambda device=None: "test device")
+    times = {(0, 0): 3.0, (0, 1): 2.0, (16, -1): 1.0, (8, -1): 1.5}
+    timed = []
+
+    def seconds(cand, device):
+        timed.append(cand)
+        return times[cand]
+
+    assert batch._tune("sched|k", "cpu", tuple(times), seconds) == (16, -1)
+    assert timed == list(times)
+
+    def no_timing(cand, device):
+        raise AssertionError("a cached key must not be timed again")
+
+    assert batch._tune("sched|k", "cpu", tuple(times), no_timing) == (16, -1)
+    monkeypatch.setattr(batch, "_cache", None)  # read back from the file
+    assert batch._tune("sched|k", "cpu", tuple(times), no_timing) == (16, -1)
+    assert batch._tune("batch|k", "cpu", (8, 16, 32), lambda c, d: abs(c - 16)) == 16
