@@ -188,12 +188,17 @@ int main(int argc, char **argv) {
                               nullptr, nullptr, 0.0f, 0.0f, nullptr, nullptr, nullptr};
     if (mode == 0) {
       for (size_t gid = 0; gid < lanes; ++gid) gather_body<float, ZV, 0>(GJ, gid);
-    } else {
+    } else if (lanes <= ((size_t)1 << 16)) {
       std::vector<unsigned char> asmem(adj_block_smem<float>(zq));
       const int gblocks = (int)((lanes + 255) / 256);
       cpu_build::run_grid(gblocks, 1, 1, 256, asmem, [&](int b, int, int, unsigned char *s) {
         gather_adj_block<float, ZV, 1>(GJ, b, s);
       });
+    } else {
+      // large problems: the per-lane search of gather_body (the same sums;
+      // the block form's barriers between 256 threads per angle would take
+      // hours here)
+      for (size_t gid = 0; gid < lanes; ++gid) gather_body<float, ZV, 1>(GJ, gid);
     }
     cpu_build::write_file(dir + (mode == 0 ? "/bp_ref.bin" : "/bp_adj.bin"), out);
   }
